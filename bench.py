@@ -1,0 +1,273 @@
+#!/usr/bin/env python3
+"""Benchmark of the sort / merge / scan hot path on one MI355X.
+
+Headline step: one stable key-value LSD radix sort (8-bit digits, 4 digit
+passes + 1 histogram pass) of 2^28 uint32 keys drawn uniform from SplitMix64
+seed 0, with uint32 values set to the original index.  Inputs are generated on
+the device and restored by a device-to-device copy before every step; each
+step's sort is bracketed by HIP events on the launch stream, so the timed
+region starts with the inputs resident in HBM and excludes the restore.
+
+The JSON line also carries:
+  roofline      the dominant kernel (one onesweep digit pass): algorithmic
+                bytes per launch (16 B/key: read key + value, write key +
+                value) / its average duration from the library's per-launch
+                HIP events, against the 8 TB/s HBM peak; `traffic` is read
+                from profiles/traffic.json when a rocprofv3 PMC measurement
+                of the same kernel has been recorded there;
+  cpu_baseline  std::stable_sort of (key, index) pairs from the CPU oracle,
+                one core, on a bounded sample of the same distribution;
+  secondary     the other primitives at their workload sizes (mergesort kv
+                2^28, merge 2 x 2^27, scans / reduce 2^29, low-entropy radix).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
+                       [--no-cpu] [--cpu-log2 L]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mgpu-shaders_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_steps(torch, steps, warmup, restore, run):
+    """W untimed steps, then K steps each timed by events around `run` only."""
+    for _ in range(warmup):
+        restore()
+        run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in ev:
+        restore()
+        e0.record()
+        run()
+        e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    return sum(ms) / len(ms), wall * 1000.0 / steps
+
+
+def kernel_times(mgpu_hip, torch, steps, restore, run):
+    """{kernel: avg ms per launch} from the library's per-launch events."""
+    restore()
+    run()
+    torch.cuda.synchronize()
+    mgpu_hip.timing_enable(True)
+    for _ in range(steps):
+        restore()
+        run()
+    torch.cuda.synchronize()
+    t = mgpu_hip.timing_collect()
+    mgpu_hip.timing_enable(False)
+    return {k: (n, tot / n) for k, (n, tot) in t.items()}
+
+
+def load_traffic(workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(log2n):
+    """std::stable_sort of (key, index) pairs on one core (the oracle)."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    import oracle
+    n = 1 << log2n
+    keys = oracle.generate(n, "u32", 0)
+    vals = np.arange(n, dtype=np.uint32)
+    t0 = time.perf_counter()
+    oracle.stable_sort(keys, vals, "u32")
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "keys/s", "cores": 1, "kind": "port",
+            "sample": "std::stable_sort of 2^%d (u32 key, u32 index) pairs, uniform keys "
+                      "(seed 0), %.2f s on one core, nproc=%d" % (log2n, dt, os.cpu_count() or 0)}
+
+
+def secondary(torch, mgpu_hip, steps, warmup):
+    out = {}
+    dev = "cuda"
+    n = 1 << 28
+
+    def gen(count, kind, seed=0, offset=0, dtype=torch.uint32):
+        t = torch.empty(count, dtype=dtype, device=dev)
+        mgpu_hip.generate(t, kind, seed, offset)
+        return t
+
+    # Low-entropy radix and mergesort, 2^28 key-value.
+    for name, kind, algo in [("radix_kv_u32_2p28_lowentropy", "lowentropy", "radix"),
+                             ("mergesort_kv_u32_2p28_uniform", "u32", "mergesort")]:
+        ks, vs = gen(n, kind), gen(n, "iota")
+        k, v = torch.empty_like(ks), torch.empty_like(vs)
+
+        def restore():
+            k.copy_(ks)
+            v.copy_(vs)
+        if algo == "radix":
+            aux = mgpu_hip.radix_sort(k, v)
+            run = lambda: mgpu_hip.radix_sort(k, v, aux=aux)
+            kname = "mgpu_radix_onesweep"
+        else:
+            aux = mgpu_hip.mergesort_kv(k, v)
+            run = lambda: mgpu_hip.mergesort_kv(k, v, aux=aux)
+            kname = "mgpu_mergesort_pass"
+        ms, _ = timed_steps(torch, steps, warmup, restore, run)
+        kt = kernel_times(mgpu_hip, torch, 2, restore, run)
+        per = kt.get(kname, (0, float("nan")))[1]
+        out[name] = {"keys_per_s": n / (ms / 1e3), "ms": ms,
+                     "pass_kernel": kname, "pass_ms": per,
+                     "pass_GBps": 16 * n / (per / 1e3) / 1e9,
+                     "pass_frac": 16 * n / (per / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "kernels_ms": {k2: round(v2[1], 4) for k2, v2 in kt.items()}}
+        log(name, out[name]["keys_per_s"] / 1e9, "Gkeys/s, pass", per, "ms")
+        del ks, vs, k, v, aux
+
+    # Merge of two sorted 2^27 uint32 arrays, key-only.
+    m = 1 << 27
+    a, b = gen(m, "u32", 1), gen(m, "u32", 2)
+    mgpu_hip.radix_sort(a)
+    mgpu_hip.radix_sort(b)
+    c = torch.empty(2 * m, dtype=torch.uint32, device=dev)
+    aux = mgpu_hip.merge_keys(a, b, c, "less")
+    run = lambda: mgpu_hip.merge_keys(a, b, c, "less", aux=aux)
+    ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+    kt = kernel_times(mgpu_hip, torch, 2, lambda: None, run)
+    per = kt.get("mgpu_merge", (0, float("nan")))[1]
+    out["merge_keys_u32_2x2p27"] = {"GBps": 8 * 2 * m / (ms / 1e3) / 1e9, "ms": ms,
+                                    "merge_kernel_ms": per,
+                                    "frac": 8 * 2 * m / (per / 1e3) / 1e9 / HBM_PEAK_GBS,
+                                    "kernels_ms": {k2: round(v2[1], 4) for k2, v2 in kt.items()}}
+    log("merge", out["merge_keys_u32_2x2p27"])
+    del a, b, c, aux
+
+    # Scans and reduces of 2^29 elements.
+    s = 1 << 29
+    xi = gen(s, "i32_0_16", 0, dtype=torch.int32)
+    yi = torch.empty_like(xi)
+    xf = gen(s, "f32_01", 0, dtype=torch.float32)
+    yf = torch.empty_like(xf)
+    r = torch.zeros(1, dtype=torch.int32, device=dev)
+    rf = torch.zeros(1, dtype=torch.float32, device=dev)
+    cases = [
+        ("scan_exclusive_i32_2p29", lambda aux: mgpu_hip.scan_into(xi, yi, aux=aux), 8 * s,
+         "mgpu_scan_lookback"),
+        ("scan_inclusive_f32_2p29",
+         lambda aux: mgpu_hip.scan_into(xf, yf, inclusive=True, aux=aux), 8 * s,
+         "mgpu_scan_lookback"),
+        ("reduce_i32_2p29", lambda aux: mgpu_hip.reduce(xi, r, aux=aux), 4 * s,
+         "mgpu_reduce_partial"),
+        ("reduce_f32_2p29", lambda aux: mgpu_hip.reduce(xf, rf, aux=aux), 4 * s,
+         "mgpu_reduce_partial"),
+    ]
+    for name, fn, nbytes, kname in cases:
+        aux = fn(None)
+        run = lambda fn=fn, aux=aux: fn(aux)
+        ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+        kt = kernel_times(mgpu_hip, torch, 2, lambda: None, run)
+        per = kt.get(kname, (0, float("nan")))[1]
+        out[name] = {"GBps": nbytes / (ms / 1e3) / 1e9, "ms": ms, "kernel_ms": per,
+                     "frac": nbytes / (per / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "kernels_ms": {k2: round(v2[1], 4) for k2, v2 in kt.items()}}
+        log(name, out[name])
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-log2", type=int, default=25)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested; run with --gpus 1)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mgpu_hip
+
+    mgpu_hip.load()
+    n = 1 << 28
+    dev = "cuda"
+    keys_src = torch.empty(n, dtype=torch.uint32, device=dev)
+    vals_src = torch.empty(n, dtype=torch.uint32, device=dev)
+    mgpu_hip.generate(keys_src, "u32", 0)
+    mgpu_hip.generate(vals_src, "iota")
+    keys = torch.empty_like(keys_src)
+    vals = torch.empty_like(vals_src)
+
+    def restore():
+        keys.copy_(keys_src)
+        vals.copy_(vals_src)
+
+    restore()
+    aux = mgpu_hip.radix_sort(keys, vals)
+    run = lambda: mgpu_hip.radix_sort(keys, vals, aux=aux)
+    log("radix kv 2^28: warmup", args.warmup, "steps", args.steps)
+    ms, wall_ms = timed_steps(torch, args.steps, args.warmup, restore, run)
+    kt = kernel_times(mgpu_hip, torch, min(args.steps, 5), restore, run)
+    log("kernel times", kt)
+    launches, pass_ms = kt["mgpu_radix_onesweep"]
+    pass_bytes = 16 * n
+    achieved = pass_bytes / (pass_ms / 1e3) / 1e9
+    workload = "radix_sort_kv_u32_2p28_uniform"
+    traffic = load_traffic(workload)
+    result = {
+        "metric": "keys/s, 2^28 uint32 key-value LSD radix sort (8-bit digits)",
+        "value": n / (ms / 1e3),
+        "unit": "keys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "wall_ms_per_step_incl_restore": wall_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (SplitMix64 seed 0, generated on device)",
+        "config": {"workload": "2^28 uint32 keys uniform over [0,2^32), uint32 values = index, "
+                               "stable LSD radix sort, 8-bit digits (4 onesweep passes + 1 "
+                               "histogram pass)",
+                   "n": n, "key": "u32", "value": "u32", "nt": 512, "vt": 16},
+        "roofline": {"bound": "hbm", "kernel": "mgpu_radix_onesweep",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "algorithmic_bytes_per_launch": pass_bytes,
+                     "avg_launch_ms": pass_ms, "launches": launches},
+        "kernels_ms": {k: round(v[1], 4) for k, v in kt.items()},
+    }
+    del keys_src, vals_src, keys, vals, aux
+    torch.cuda.empty_cache()
+    if not args.no_secondary:
+        result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
+        torch.cuda.empty_cache()
+    if not args.no_cpu:
+        log("cpu baseline 2^%d" % args.cpu_log2)
+        result["cpu_baseline"] = cpu_baseline(args.cpu_log2)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,171 @@
+/*
+ * mgpu_hip.h -- C ABI of the MI355X (gfx950) sort / merge / scan primitives.
+ *
+ * Every entry point keeps the reference's two-phase auxiliary-memory protocol
+ * (inc/mgpu/vk/mergesort.hxx:35-41, inc/mgpu/vk/merge.hxx:30-34,
+ * inc/mgpu/vk/scan.hxx:54-58, inc/mgpu/vk/radix.hxx:288-302):
+ *
+ *   1. Call with aux == NULL: the required byte count is ADDED to *aux_size
+ *      (accumulating, so one pool can be sized for several calls) and nothing
+ *      is launched.
+ *   2. Call again with aux != NULL and *aux_size == capacity of aux: the work
+ *      is enqueued on `stream` (asynchronous, nothing is read back to the
+ *      host).  A capacity below the requirement returns MGPU_AUX_TOO_SMALL
+ *      (the reference never checked; see inc/mgpu/vk/mergesort.hxx:43-46).
+ *
+ * As in the reference, a call that needs zero aux bytes still returns early
+ * when aux is NULL, so callers pass any non-NULL pointer (a 1-byte buffer)
+ * for the second call.  count == 0 is a defined no-op (the reference's
+ * find_log2(0) is undefined, inc/mgpu/common/meta.hxx:37-41).
+ *
+ * All pointers are device pointers (hipMalloc / torch tensors).  Vectorised
+ * 128-bit paths are taken when pointers are 16-byte aligned, otherwise a
+ * scalar path runs; results are identical.  Threading: the library holds no
+ * mutable global state on the compute path; calls on different streams are
+ * independent.  No C++ exception crosses this boundary.
+ */
+#ifndef MGPU_HIP_H
+#define MGPU_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without pulling in the HIP headers (same opaque type). */
+typedef struct ihipStream_t* mgpu_stream_t;
+
+typedef enum mgpu_status_t {
+  MGPU_OK = 0,
+  MGPU_INVALID_ARG = 1,   /* NULL pointer with count > 0, negative count, bad enum */
+  MGPU_AUX_TOO_SMALL = 2, /* *aux_size (capacity) below the requirement          */
+  MGPU_HIP_ERROR = 3,     /* a HIP runtime call failed; see mgpu_last_hip_error  */
+  MGPU_UNSUPPORTED = 4    /* size beyond a documented limit of the entry point   */
+} mgpu_status_t;
+
+/* Comparator for mergesort / merge (inc/mgpu/vk/mergesort.hxx:11 takes any
+ * functor; the C ABI exports the two standard ones, custom functors use the
+ * C++ templates in mgpu/hip/ headers). */
+typedef enum mgpu_comp_t { MGPU_LESS = 0, MGPU_GREATER = 1 } mgpu_comp_t;
+
+typedef enum mgpu_scan_type_t { MGPU_SCAN_EXCLUSIVE = 0, MGPU_SCAN_INCLUSIVE = 1 } mgpu_scan_type_t;
+
+const char* mgpu_status_string(mgpu_status_t status);
+/* hipError_t of the last MGPU_HIP_ERROR raised on the calling thread. */
+int mgpu_last_hip_error(void);
+/* Library version: major*10000 + minor*100 + patch. */
+int mgpu_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Mergesort: stable, in place.  Replaces vk::mergesort_keys / mergesort_kv /
+ * mergesort_indices (inc/mgpu/vk/mergesort.hxx:86-93, 10-84, 95-103).
+ * aux: ping-pong keys (+ values) and merge-path partitions.
+ * Key types: u32, i32, u64, f32.  Values: u32.  indices: vals[i] := i before
+ * the sort (inc/mgpu/common/kernel_mergesort.hxx:43-49).  count < 2^31.     */
+#define MGPU_DECL_MERGESORT(K, KT)                                             \
+  mgpu_status_t mgpu_mergesort_keys_##K(void* aux, size_t* aux_size,           \
+      mgpu_stream_t stream, KT* keys, int64_t count, mgpu_comp_t comp);        \
+  mgpu_status_t mgpu_mergesort_kv_##K##_u32(void* aux, size_t* aux_size,       \
+      mgpu_stream_t stream, KT* keys, uint32_t* vals, int64_t count,           \
+      mgpu_comp_t comp);                                                       \
+  mgpu_status_t mgpu_mergesort_indices_##K(void* aux, size_t* aux_size,        \
+      mgpu_stream_t stream, KT* keys, int32_t* indices, int64_t count,         \
+      mgpu_comp_t comp);
+MGPU_DECL_MERGESORT(u32, uint32_t)
+MGPU_DECL_MERGESORT(i32, int32_t)
+MGPU_DECL_MERGESORT(u64, uint64_t)
+MGPU_DECL_MERGESORT(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* Two-way stable merge (ties take A first), out of place into c.
+ * Replaces vk::merge key-value and key-only (inc/mgpu/vk/merge.hxx:12-46,
+ * 49-59) together with vk::merge_path_partitions (inc/mgpu/vk/partition.hxx:
+ * 9-20).  aux: one int64 partition per output tile.                         */
+#define MGPU_DECL_MERGE(K, KT)                                                 \
+  mgpu_status_t mgpu_merge_keys_##K(void* aux, size_t* aux_size,               \
+      mgpu_stream_t stream, const KT* a_keys, int64_t a_count,                 \
+      const KT* b_keys, int64_t b_count, KT* c_keys, mgpu_comp_t comp);        \
+  mgpu_status_t mgpu_merge_kv_##K##_u32(void* aux, size_t* aux_size,           \
+      mgpu_stream_t stream, const KT* a_keys, const uint32_t* a_vals,          \
+      int64_t a_count, const KT* b_keys, const uint32_t* b_vals,               \
+      int64_t b_count, KT* c_keys, uint32_t* c_vals, mgpu_comp_t comp);
+MGPU_DECL_MERGE(u32, uint32_t)
+MGPU_DECL_MERGE(i32, int32_t)
+MGPU_DECL_MERGE(u64, uint64_t)
+MGPU_DECL_MERGE(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* LSD radix sort, 8-bit digits, ascending in radix-bit order (unsigned:
+ * value order; signed: MSB flipped; float: -0.0 before +0.0, NaNs last),
+ * stable, in place.  Replaces vk::radix_sort (inc/mgpu/vk/radix.hxx:195-470)
+ * and extends it with values.  Bit order per inc/mgpu/common/cta_radix.hxx:
+ * 6-64.  count < 2^30 (30-bit look-back counters).                          */
+#define MGPU_DECL_RADIX(K, KT)                                                 \
+  mgpu_status_t mgpu_radix_sort_keys_##K(void* aux, size_t* aux_size,          \
+      mgpu_stream_t stream, KT* keys, int64_t count);                          \
+  mgpu_status_t mgpu_radix_sort_kv_##K##_u32(void* aux, size_t* aux_size,      \
+      mgpu_stream_t stream, KT* keys, uint32_t* vals, int64_t count);
+MGPU_DECL_RADIX(u32, uint32_t)
+MGPU_DECL_RADIX(i32, int32_t)
+MGPU_DECL_RADIX(u64, uint64_t)
+MGPU_DECL_RADIX(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* Scan with plus.
+ * mgpu_scan_exclusive_T: in place, data[i] := init + sum(data[0..i)) -- the
+ *   exact counterpart of vk::scan (inc/mgpu/vk/scan.hxx:10-115), with init
+ *   applied once (std::exclusive_scan semantics).
+ * mgpu_scan_T: out of place or in place (input == output), exclusive or
+ *   inclusive.
+ * Integer types wrap modulo 2^32 (two's complement).  f32: hierarchical
+ *   float32 sums inside a 4096-element tile, float64 carries between tiles.
+ * mgpu_reduce_T: *out (device) := init + sum(input[0..count)); f32 sums in
+ *   float64 and rounds once.                                                */
+#define MGPU_DECL_SCAN(T, TT)                                                  \
+  mgpu_status_t mgpu_scan_exclusive_##T(void* aux, size_t* aux_size,           \
+      mgpu_stream_t stream, TT* data, int64_t count, TT init);                 \
+  mgpu_status_t mgpu_scan_##T(void* aux, size_t* aux_size,                     \
+      mgpu_stream_t stream, const TT* input, TT* output, int64_t count,        \
+      mgpu_scan_type_t type, TT init);                                         \
+  mgpu_status_t mgpu_reduce_##T(void* aux, size_t* aux_size,                   \
+      mgpu_stream_t stream, const TT* input, int64_t count, TT* out, TT init);
+MGPU_DECL_SCAN(i32, int32_t)
+MGPU_DECL_SCAN(u32, uint32_t)
+MGPU_DECL_SCAN(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic inputs (no counterpart in the reference, whose examples fill
+ * host buffers with rand(), e.g. examples/vk_radix2/vk_radix2.cxx:25-27).
+ * Counter-based SplitMix64: element i of `kind` for `seed`, written for
+ * i in [0, count) as element (offset + i) of the stream.  elem_bytes is 4 or
+ * 8 (kind MGPU_GEN_U64 needs 8).                                           */
+typedef enum mgpu_gen_kind_t {
+  MGPU_GEN_U32 = 0,        /* uniform over [0, 2^32)                         */
+  MGPU_GEN_LOWENTROPY = 1, /* AND of four uniform u32 draws                  */
+  MGPU_GEN_I32_0_16 = 2,   /* uniform int32 in [0, 16)                       */
+  MGPU_GEN_F32_01 = 3,     /* float32 multiples of 2^-24 in [0, 1)           */
+  MGPU_GEN_U64 = 4,        /* uniform over [0, 2^64)                         */
+  MGPU_GEN_F32_PM1 = 5,    /* float32 multiples of 2^-23 in [-1, 1)          */
+  MGPU_GEN_IOTA = 6        /* element i = i                                  */
+} mgpu_gen_kind_t;
+mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
+                            mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing (HIP events recorded on the launch stream around every
+ * kernel the library enqueues while enabled).  Used by benchmarks to get a
+ * per-kernel average duration.  Not thread-safe while enabled.            */
+void mgpu_timing_enable(int enable);
+/* Synchronises the recorded events and returns the number of distinct kernel
+ * names; for i < capacity fills names[i] (pointer valid until the next
+ * mgpu_timing_enable call), launches[i] and total_ms[i].                   */
+int mgpu_timing_collect(const char** names, int* launches, double* total_ms,
+                        int capacity);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* MGPU_HIP_H */

@@ -1,0 +1,80 @@
+// C ABI: status strings, HIP error slot, event timing, synthetic generator.
+#include <map>
+
+#include "mgpu/hip/generate.hxx"
+#include "mgpu/hip/meta.hxx"
+
+using namespace mgpu::hip;
+
+extern "C" {
+
+const char* mgpu_status_string(mgpu_status_t status) {
+  switch (status) {
+    case MGPU_OK: return "MGPU_OK";
+    case MGPU_INVALID_ARG: return "MGPU_INVALID_ARG";
+    case MGPU_AUX_TOO_SMALL: return "MGPU_AUX_TOO_SMALL";
+    case MGPU_HIP_ERROR: return "MGPU_HIP_ERROR";
+    case MGPU_UNSUPPORTED: return "MGPU_UNSUPPORTED";
+  }
+  return "MGPU_UNKNOWN_STATUS";
+}
+
+int mgpu_last_hip_error(void) { return last_hip_error_slot(); }
+
+int mgpu_version(void) { return 100; }
+
+void mgpu_timing_enable(int enable) {
+  timing_registry_t& tr = timing_registry();
+  std::lock_guard<std::mutex> lock(tr.mutex);
+  for (auto& r : tr.recs) {
+    tr.pool.push_back(r.start);
+    tr.pool.push_back(r.stop);
+  }
+  tr.recs.clear();
+  tr.enabled = enable != 0;
+}
+
+int mgpu_timing_collect(const char** names, int* launches, double* total_ms, int capacity) {
+  timing_registry_t& tr = timing_registry();
+  std::lock_guard<std::mutex> lock(tr.mutex);
+  std::map<std::string, std::pair<int, double>> agg;
+  std::vector<std::string> order;
+  for (auto& r : tr.recs) {
+    (void)hipEventSynchronize(r.stop);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r.start, r.stop);
+    std::string key(r.name);
+    auto it = agg.find(key);
+    if (it == agg.end()) {
+      agg[key] = {1, (double)ms};
+      order.push_back(key);
+    } else {
+      it->second.first += 1;
+      it->second.second += ms;
+    }
+  }
+  tr.names = order;
+  int n = (int)order.size();
+  for (int i = 0; i < n && i < capacity; ++i) {
+    if (names) names[i] = tr.names[i].c_str();
+    if (launches) launches[i] = agg[order[i]].first;
+    if (total_ms) total_ms[i] = agg[order[i]].second;
+  }
+  return n;
+}
+
+mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
+                            mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset) {
+  if (count < 0 || (elem_bytes != 4 && elem_bytes != 8)) return MGPU_INVALID_ARG;
+  if ((int)kind < 0 || (int)kind > (int)gen_iota) return MGPU_INVALID_ARG;
+  if (kind == MGPU_GEN_U64 && elem_bytes != 8) return MGPU_INVALID_ARG;
+  if (count == 0) return MGPU_OK;
+  if (!out) return MGPU_INVALID_ARG;
+  int64_t blocks = min_<int64_t>(div_up(count, (int64_t)256), 8192);
+  hipError_t e = launch_kernel("mgpu_generate", kernel_generate<256>, dim3((unsigned)blocks),
+                               dim3(256), 0, (hipStream_t)stream, out, count, elem_bytes,
+                               (int)kind, seed, offset);
+  return hip_status(e);
+}
+
+}  // extern "C"

@@ -1,0 +1,26 @@
+// C ABI: LSD radix sort (inc/mgpu/vk/radix.hxx:195-470 counterpart, plus values).
+#include "mgpu/hip/radix.hxx"
+#include "tuning.hxx"
+
+using namespace mgpu::hip;
+
+#define MGPU_DEF_RADIX(K, KT)                                                               \
+  extern "C" mgpu_status_t mgpu_radix_sort_keys_##K(void* aux, size_t* aux_size,            \
+                                                    mgpu_stream_t stream, KT* keys,         \
+                                                    int64_t count) {                        \
+    if (!aux_size) return MGPU_INVALID_ARG;                                                 \
+    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt;                     \
+    return radix_sort<NT, VT>(aux, *aux_size, (hipStream_t)stream, keys, count);            \
+  }                                                                                         \
+  extern "C" mgpu_status_t mgpu_radix_sort_kv_##K##_u32(void* aux, size_t* aux_size,        \
+                                                        mgpu_stream_t stream, KT* keys,     \
+                                                        uint32_t* vals, int64_t count) {    \
+    if (!aux_size) return MGPU_INVALID_ARG;                                                 \
+    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt;                     \
+    return radix_sort<NT, VT>(aux, *aux_size, (hipStream_t)stream, keys, vals, count);      \
+  }
+
+MGPU_DEF_RADIX(u32, uint32_t)
+MGPU_DEF_RADIX(i32, int32_t)
+MGPU_DEF_RADIX(u64, uint64_t)
+MGPU_DEF_RADIX(f32, float)

@@ -1,0 +1,73 @@
+// Counter-based SplitMix64 input generators (Steele, Lea & Flood 2014), the
+// same formulas on host and device so both produce identical inputs:
+//   word(seed, stream, i) = mix64(seed * 2^40 + stream * 2^36 + i + 0x9E3779B97F4A7C15)
+//   u32(stream, i)        = word(...) >> 32
+// Kinds:
+//   u32         u32(0, i)
+//   lowentropy  u32(0,i) & u32(1,i) & u32(2,i) & u32(3,i)
+//   i32_0_16    u32(0, i) >> 28
+//   f32_01      (u32(0, i) >> 8) * 2^-24          (exact multiples of 2^-24)
+//   u64         u32(0, i) << 32 | u32(1, i)
+//   f32_pm1     (u32(0, i) >> 8) * 2^-23 - 1      (exact, in [-1, 1))
+//   iota        i
+#pragma once
+#include "meta.hxx"
+
+namespace mgpu {
+namespace hip {
+
+enum gen_kind_t : int {
+  gen_u32 = 0,
+  gen_lowentropy = 1,
+  gen_i32_0_16 = 2,
+  gen_f32_01 = 3,
+  gen_u64 = 4,
+  gen_f32_pm1 = 5,
+  gen_iota = 6,
+};
+
+__host__ __device__ inline uint64_t splitmix64_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint32_t gen_word32(uint64_t seed, uint64_t stream, uint64_t i) {
+  uint64_t c = (seed << 40) + (stream << 36) + i + 0x9E3779B97F4A7C15ull;
+  return (uint32_t)(splitmix64_mix(c) >> 32);
+}
+
+// Element i of a generator kind, as raw bits (4 or 8 bytes).
+__host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_t i) {
+  switch (kind) {
+    case gen_u32: return gen_word32(seed, 0, i);
+    case gen_lowentropy:
+      return gen_word32(seed, 0, i) & gen_word32(seed, 1, i) & gen_word32(seed, 2, i) &
+             gen_word32(seed, 3, i);
+    case gen_i32_0_16: return gen_word32(seed, 0, i) >> 28;
+    case gen_f32_01: {
+      float f = (float)(gen_word32(seed, 0, i) >> 8) * (1.0f / 16777216.0f);
+      uint32_t b; std::memcpy(&b, &f, 4); return b;
+    }
+    case gen_u64: return ((uint64_t)gen_word32(seed, 0, i) << 32) | gen_word32(seed, 1, i);
+    case gen_f32_pm1: {
+      float f = (float)(gen_word32(seed, 0, i) >> 8) * (1.0f / 8388608.0f) - 1.0f;
+      uint32_t b; std::memcpy(&b, &f, 4); return b;
+    }
+    case gen_iota: return i;
+    default: return 0;
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void kernel_generate(void* out, int64_t count, int elem_bytes,
+                                                      int kind, uint64_t seed, uint64_t offset) {
+  int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < count; i += stride) {
+    uint64_t v = gen_element(kind, seed, offset + (uint64_t)i);
+    if (elem_bytes == 8) ((uint64_t*)out)[i] = v;
+    else ((uint32_t*)out)[i] = (uint32_t)v;
+  }
+}
+
+}  // namespace hip
+}  // namespace mgpu

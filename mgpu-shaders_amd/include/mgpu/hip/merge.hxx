@@ -1,0 +1,242 @@
+// Merge-path primitives and the two-way stable merge for gfx950.
+//
+// merge_path follows inc/mgpu/common/cta_merge.hxx:10-31 with bounds_lower
+// (ties go to A): it returns the number of A elements among the first `diag`
+// outputs of the stable merge.  The partition search that the reference runs
+// as one binary search per thread in global memory (vk/partition.hxx:9-20) is
+// done here by a whole wave: 64 lanes probe 64 evenly spaced points, so a
+// 2^27-element range resolves in 5 dependent rounds instead of 27.
+//
+// A tile merge (kernel_merge.hxx:9-58 and cta_merge.hxx:136-162) stages the A
+// and B slices of one output tile in LDS with coalesced loads, runs one merge
+// path per thread and a vt-step serial merge (cta_merge.hxx:110-133), and
+// stores keys and values through LDS; values are gathered from LDS rather than
+// from global memory (the reference gathers from global, cta_merge.hxx:84-105).
+#pragma once
+#include "meta.hxx"
+
+namespace mgpu {
+namespace hip {
+
+// Per-thread merge path (binary search), used in LDS.
+template <typename key_t, typename comp_t>
+__device__ inline int merge_path_lds(const key_t* a, int a_count, const key_t* b,
+                                     int b_count, int diag, comp_t comp) {
+  int begin = max_(0, diag - b_count);
+  int end = min_(diag, a_count);
+  while (begin < end) {
+    int mid = (begin + end) >> 1;
+    bool pred = !comp(b[diag - 1 - mid], a[mid]);
+    if (pred) begin = mid + 1;
+    else end = mid;
+  }
+  return begin;
+}
+
+// Wave-cooperative merge path in global memory.  All 64 lanes must call it
+// with the same arguments; every lane returns the result.
+template <typename key_t, typename comp_t>
+__device__ inline int64_t merge_path_wave(const key_t* a, int64_t a_count, const key_t* b,
+                                          int64_t b_count, int64_t diag, comp_t comp) {
+  const int lane = lane_id();
+  int64_t begin = max_<int64_t>(0, diag - b_count);
+  int64_t end = min_<int64_t>(diag, a_count);
+  while (begin < end) {
+    int64_t n = end - begin;
+    int64_t step = n <= warp_size ? 1 : div_up(n, (int64_t)warp_size);
+    int64_t s = begin + (int64_t)lane * step;
+    bool valid = s < end;
+    bool pred = false;
+    if (valid) pred = !comp(b[diag - 1 - s], a[s]);
+    uint64_t bal = __ballot(pred);
+    uint64_t vmask = __ballot(valid);
+    int cnt = __popcll(bal);
+    int nvalid = __popcll(vmask);
+    // pred is true exactly for probes below the answer.
+    if (step == 1) return begin + cnt;
+    int64_t nb = cnt == 0 ? begin : begin + (int64_t)(cnt - 1) * step + 1;
+    int64_t ne = cnt < nvalid ? begin + (int64_t)cnt * step : end;
+    begin = nb;
+    end = ne;
+  }
+  return begin;
+}
+
+// Serial merge of vt outputs starting at (a_begin, b_begin) of LDS keys, with
+// the range checks of merge_predicate (cta_merge.hxx:41-50).  keys_shared
+// must be readable one element past the ranges.
+template <int VT, typename key_t, typename comp_t>
+__device__ inline void serial_merge(const key_t* keys_shared, int a_begin, int a_end,
+                                    int b_begin, int b_end, key_t (&keys)[VT],
+                                    int (&indices)[VT], comp_t comp) {
+  key_t a_key = keys_shared[a_begin];
+  key_t b_key = keys_shared[b_begin];
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    bool p;
+    if (a_begin >= a_end) p = false;
+    else if (b_begin >= b_end) p = true;
+    else p = !comp(b_key, a_key);
+    int index = p ? a_begin : b_begin;
+    keys[i] = p ? a_key : b_key;
+    indices[i] = index;
+    key_t c_key = keys_shared[index + 1];
+    if (p) { a_key = c_key; a_begin = index + 1; }
+    else { b_key = c_key; b_begin = index + 1; }
+  }
+}
+
+// Partition kernel: mp[i] = merge_path(a, b, min(i * spacing, a_count + b_count)).
+// One wave per partition.
+template <int NT, typename key_t, typename comp_t>
+__global__ __launch_bounds__(NT) void kernel_merge_partition(
+    const key_t* a, int64_t a_count, const key_t* b, int64_t b_count, int64_t spacing,
+    int64_t num_partitions, int64_t* mp, comp_t comp) {
+  int64_t index = (int64_t)blockIdx.x * (NT / warp_size) + threadIdx.x / warp_size;
+  if (index >= num_partitions) return;
+  int64_t diag = min_<int64_t>(spacing * index, a_count + b_count);
+  int64_t r = merge_path_wave(a, a_count, b, b_count, diag, comp);
+  if (lane_id() == 0) mp[index] = r;
+}
+
+// LDS footprint of a tile merge.
+template <int NT, int VT, typename key_t, typename val_t>
+struct merge_storage_t {
+  static constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  key_t keys[NT * VT + 1];
+  typename std::conditional<has_values, val_t, char>::type vals[has_values ? NT * VT + 1 : 1];
+};
+
+// Merge A[a_begin, a_begin+a_cnt) and B[b_begin, b_begin+b_cnt) (a_cnt+b_cnt <=
+// NT*VT) into C[c_begin, ...), keys and values.  Called by the whole block.
+template <int NT, int VT, typename key_t, typename val_t, typename comp_t>
+__device__ inline void cta_merge_tile(
+    const key_t* a_keys, const val_t* a_vals, int64_t a_begin, int a_cnt,
+    const key_t* b_keys, const val_t* b_vals, int64_t b_begin, int b_cnt,
+    key_t* c_keys, val_t* c_vals, int64_t c_begin, int c_count, comp_t comp,
+    merge_storage_t<NT, VT, key_t, val_t>& s) {
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  const int tid = threadIdx.x;
+  // a_cnt + b_cnt == c_count <= NT*VT for valid partitions; the clamps keep a
+  // corrupt partition from reaching outside the tile.
+  const int total = min_(min_(a_cnt + b_cnt, c_count), NT * VT);
+  a_cnt = min_(a_cnt, total);
+  b_cnt = total - a_cnt;
+
+  // Load both streams into LDS (coalesced strided loads).
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    int j = i * NT + tid;
+    if (j < a_cnt) s.keys[j] = a_keys[a_begin + j];
+    else if (j < total) s.keys[j] = b_keys[b_begin + (j - a_cnt)];
+  }
+  if constexpr (has_values) {
+#pragma unroll
+    for (int i = 0; i < VT; ++i) {
+      int j = i * NT + tid;
+      if (j < a_cnt) s.vals[j] = a_vals[a_begin + j];
+      else if (j < total) s.vals[j] = b_vals[b_begin + (j - a_cnt)];
+    }
+  }
+  __syncthreads();
+
+  int diag = min_(VT * tid, total);
+  int mp = merge_path_lds(s.keys, a_cnt, s.keys + a_cnt, b_cnt, diag, comp);
+  key_t keys[VT];
+  int indices[VT];
+  serial_merge<VT>(s.keys, mp, a_cnt, a_cnt + diag - mp, total, keys, indices, comp);
+  val_t vals[VT];
+  if constexpr (has_values) {
+#pragma unroll
+    for (int i = 0; i < VT; ++i) vals[i] = s.vals[indices[i]];
+  }
+  __syncthreads();
+
+  // Thread order -> LDS -> strided coalesced stores.
+#pragma unroll
+  for (int i = 0; i < VT; ++i) s.keys[VT * tid + i] = keys[i];
+  if constexpr (has_values) {
+#pragma unroll
+    for (int i = 0; i < VT; ++i) s.vals[VT * tid + i] = vals[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    int j = i * NT + tid;
+    if (j < total) c_keys[c_begin + j] = s.keys[j];
+  }
+  if constexpr (has_values) {
+#pragma unroll
+    for (int i = 0; i < VT; ++i) {
+      int j = i * NT + tid;
+      if (j < total) c_vals[c_begin + j] = s.vals[j];
+    }
+  }
+}
+
+template <int NT, int VT, typename key_t, typename val_t, typename comp_t>
+__global__ __launch_bounds__(NT) void kernel_merge(
+    const int64_t* mp, const key_t* a_keys, const val_t* a_vals, int64_t a_count,
+    const key_t* b_keys, const val_t* b_vals, int64_t b_count, key_t* c_keys, val_t* c_vals,
+    comp_t comp) {
+  constexpr int NV = NT * VT;
+  __shared__ merge_storage_t<NT, VT, key_t, val_t> s;
+  const int64_t cta = blockIdx.x;
+  const int64_t diag0 = cta * NV;
+  const int64_t diag1 = min_<int64_t>(a_count + b_count, diag0 + NV);
+  const int64_t mp0 = mp[cta], mp1 = mp[cta + 1];
+  cta_merge_tile<NT, VT>(a_keys, a_vals, mp0, (int)(mp1 - mp0), b_keys, b_vals, diag0 - mp0,
+                         (int)((diag1 - mp1) - (diag0 - mp0)), c_keys, c_vals, diag0,
+                         (int)(diag1 - diag0), comp, s);
+}
+
+// Stable merge of (a_keys, a_vals) and (b_keys, b_vals) into (c_keys, c_vals);
+// val_t = empty_t for key-only.  inc/mgpu/vk/merge.hxx:12-59.
+template <int NT = 256, int VT = 16, typename key_t, typename val_t, typename comp_t>
+mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
+                    const key_t* a_keys, const val_t* a_vals, int64_t a_count,
+                    const key_t* b_keys, const val_t* b_vals, int64_t b_count,
+                    key_t* c_keys, val_t* c_vals, comp_t comp) {
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  constexpr int NV = NT * VT;
+  if (a_count < 0 || b_count < 0) return MGPU_INVALID_ARG;
+  const int64_t total = a_count + b_count;
+  if (total == 0) return MGPU_OK;
+  if ((a_count && !a_keys) || (b_count && !b_keys) || !c_keys) return MGPU_INVALID_ARG;
+  if (has_values && ((a_count && !a_vals) || (b_count && !b_vals) || !c_vals))
+    return MGPU_INVALID_ARG;
+  const int64_t num_tiles = div_up(total, (int64_t)NV);
+  if (num_tiles >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
+  const int64_t num_partitions = num_tiles + 1;
+  size_t need = aux_bytes<int64_t>(num_partitions);
+  if (!aux_data) {
+    aux_size += need;
+    return MGPU_OK;
+  }
+  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  void* p = aux_data;
+  int64_t* mp = advance_pointer<int64_t>(p, num_partitions);
+  constexpr int PNT = 256;
+  hipError_t e = launch_kernel("mgpu_merge_partition", kernel_merge_partition<PNT, key_t, comp_t>,
+                               dim3((unsigned)div_up(num_partitions, (int64_t)(PNT / warp_size))),
+                               dim3(PNT), 0, stream, a_keys, a_count, b_keys, b_count,
+                               (int64_t)NV, num_partitions, mp, comp);
+  if (e != hipSuccess) return hip_status(e);
+  e = launch_kernel("mgpu_merge", kernel_merge<NT, VT, key_t, val_t, comp_t>,
+                    dim3((unsigned)num_tiles), dim3(NT), 0, stream, (const int64_t*)mp, a_keys,
+                    a_vals, a_count, b_keys, b_vals, b_count, c_keys, c_vals, comp);
+  return hip_status(e);
+}
+
+// Key-only merge (inc/mgpu/vk/merge.hxx:49-59; an explicit comparator).
+template <int NT = 256, int VT = 16, typename key_t, typename comp_t>
+mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
+                    const key_t* a_keys, int64_t a_count, const key_t* b_keys,
+                    int64_t b_count, key_t* c_keys, comp_t comp) {
+  return merge<NT, VT>(aux_data, aux_size, stream, a_keys, (const empty_t*)nullptr, a_count,
+                       b_keys, (const empty_t*)nullptr, b_count, c_keys, (empty_t*)nullptr,
+                       comp);
+}
+
+}  // namespace hip
+}  // namespace mgpu

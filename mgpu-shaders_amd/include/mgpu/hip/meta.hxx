@@ -1,0 +1,275 @@
+// Common helpers for the gfx950 primitives: integer helpers, comparators,
+// key <-> radix-bit maps, aux-buffer carving, kernel launch with optional
+// event timing, and wave64 primitives (ballot, DPP scans).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "mgpu_hip.h"
+
+#ifndef __HIP_PLATFORM_AMD__
+#error "mgpu-hip targets AMD gfx950 only"
+#endif
+
+namespace mgpu {
+namespace hip {
+
+constexpr int warp_size = 64;
+
+struct empty_t {};
+
+// div_up / is_pow2 / find_log2: inc/mgpu/common/meta.hxx:20-41, extended to
+// 64-bit and with find_log2(0) defined as 0 (the reference's is undefined).
+template <typename a_t, typename b_t>
+__host__ __device__ constexpr a_t div_up(a_t x, b_t y) {
+  return (x + (a_t)y - 1) / (a_t)y;
+}
+__host__ __device__ constexpr bool is_pow2(int64_t x) {
+  return x > 0 && 0 == (x & (x - 1));
+}
+__host__ __device__ constexpr int find_log2(int64_t x, bool round_up = false) {
+  if (x <= 0) return 0;
+  int a = 63 - __builtin_clzll((unsigned long long)x);
+  if (round_up) a += !is_pow2(x);
+  return a;
+}
+template <typename T>
+__host__ __device__ constexpr T min_(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__host__ __device__ constexpr T max_(T a, T b) { return a < b ? b : a; }
+
+// Standard comparators (the reference defaults to std::less<key_t>,
+// inc/mgpu/vk/mergesort.hxx:11).
+template <typename T>
+struct less_t {
+  __host__ __device__ bool operator()(T a, T b) const { return a < b; }
+};
+template <typename T>
+struct greater_t {
+  __host__ __device__ bool operator()(T a, T b) const { return b < a; }
+};
+
+////////////////////////////////////////////////////////////////////////////////
+// Key <-> order-preserving unsigned bits, as radix_permute_t in
+// inc/mgpu/common/cta_radix.hxx:6-64 (the signed flip uses a shift of the
+// unsigned type, so 64-bit signed keys are well defined).
+
+template <int size> struct uint_by_size;
+template <> struct uint_by_size<4> { typedef uint32_t type; typedef int32_t stype; };
+template <> struct uint_by_size<8> { typedef uint64_t type; typedef int64_t stype; };
+
+template <typename key_t>
+struct radix_permute_t {
+  typedef typename uint_by_size<sizeof(key_t)>::type unsigned_type;
+  typedef typename uint_by_size<sizeof(key_t)>::stype signed_type;
+  static constexpr unsigned_type msb = (unsigned_type)1 << (8 * sizeof(key_t) - 1);
+
+  __host__ __device__ static unsigned_type to_radix_bits(key_t x) {
+    unsigned_type y;
+    std::memcpy(&y, &x, sizeof(y));
+    if constexpr (std::is_unsigned_v<key_t>) {
+      return y;
+    } else if constexpr (std::is_integral_v<key_t>) {
+      return y ^ msb;
+    } else {
+      unsigned_type mask = (unsigned_type)((signed_type)y >> (8 * sizeof(key_t) - 1)) | msb;
+      return y ^ mask;
+    }
+  }
+  __host__ __device__ static key_t from_radix_bits(unsigned_type y) {
+    if constexpr (std::is_unsigned_v<key_t>) {
+      key_t x; std::memcpy(&x, &y, sizeof(x)); return x;
+    } else if constexpr (std::is_integral_v<key_t>) {
+      y ^= msb;
+      key_t x; std::memcpy(&x, &y, sizeof(x)); return x;
+    } else {
+      y ^= msb;
+      unsigned_type mask = (unsigned_type)((signed_type)y >> (8 * sizeof(key_t) - 1)) & (msb - 1);
+      y ^= mask;
+      key_t x; std::memcpy(&x, &y, sizeof(x)); return x;
+    }
+  }
+};
+
+////////////////////////////////////////////////////////////////////////////////
+// Aux carving (advance_pointer, inc/mgpu/common/meta.hxx:119-124).  Every
+// region is rounded up to 256 bytes so that 128-bit loads stay aligned.
+
+constexpr size_t aux_align = 256;
+__host__ inline size_t aux_round(size_t bytes) {
+  return (bytes + aux_align - 1) / aux_align * aux_align;
+}
+template <typename T>
+__host__ inline size_t aux_bytes(int64_t count) {
+  return aux_round(sizeof(T) * (size_t)(count > 0 ? count : 0));
+}
+template <typename T>
+__host__ inline T* advance_pointer(void*& p, int64_t count) {
+  T* r = (T*)p;
+  p = (char*)p + aux_bytes<T>(count);
+  return r;
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Per-thread last HIP error (reported through mgpu_last_hip_error).
+
+inline int& last_hip_error_slot() {
+  static thread_local int e = 0;
+  return e;
+}
+inline mgpu_status_t hip_status(hipError_t e) {
+  if (e == hipSuccess) return MGPU_OK;
+  last_hip_error_slot() = (int)e;
+  return MGPU_HIP_ERROR;
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Event timing of every launch while enabled.
+
+struct timing_registry_t {
+  std::mutex mutex;
+  bool enabled = false;
+  struct rec_t { const char* name; hipEvent_t start, stop; };
+  std::vector<rec_t> recs;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> names;  // storage for collected names
+
+  hipEvent_t get_event() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+};
+inline timing_registry_t& timing_registry() {
+  static timing_registry_t r;
+  return r;
+}
+
+template <typename kernel_t, typename... args_t>
+inline hipError_t launch_kernel(const char* name, kernel_t kernel, dim3 grid,
+                                dim3 block, size_t shmem, hipStream_t stream,
+                                args_t... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return hipSuccess;
+  timing_registry_t& tr = timing_registry();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (tr.enabled) {
+    std::lock_guard<std::mutex> lock(tr.mutex);
+    e0 = tr.get_event();
+    e1 = tr.get_event();
+    (void)hipEventRecord(e0, stream);
+  }
+  hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+  hipError_t err = hipGetLastError();
+  if (tr.enabled) {
+    std::lock_guard<std::mutex> lock(tr.mutex);
+    (void)hipEventRecord(e1, stream);
+    tr.recs.push_back({name, e0, e1});
+  }
+  return err;
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// wave64 primitives.
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+
+// Number of set bits of `mask` in lanes below the calling lane.
+__device__ inline uint32_t mask_count_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Inclusive wave scan of a 32-bit value with DPP: row_shr 1,2,4,8 inside each
+// row of 16 lanes, then row_bcast:15 and row_bcast:31 carry across rows.
+// Lanes whose DPP source is out of range read the identity 0.
+__device__ inline uint32_t wave_scan_incl_u32(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+  return x;
+}
+__device__ inline float wave_scan_incl_f32(float x) {
+#define MGPU_DPP_F(ctrl, rmask) \
+  __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, rmask, 0xf, false))
+  x += MGPU_DPP_F(0x111, 0xf);
+  x += MGPU_DPP_F(0x112, 0xf);
+  x += MGPU_DPP_F(0x114, 0xf);
+  x += MGPU_DPP_F(0x118, 0xf);
+  x += MGPU_DPP_F(0x142, 0xa);
+  x += MGPU_DPP_F(0x143, 0xc);
+#undef MGPU_DPP_F
+  return x;
+}
+__device__ inline uint32_t wave_scan_incl(uint32_t x) { return wave_scan_incl_u32(x); }
+__device__ inline float wave_scan_incl(float x) { return wave_scan_incl_f32(x); }
+
+// Value of the previous lane (lane 0 receives `first`).
+template <typename T>
+__device__ inline T wave_shift_up1(T x, T first) {
+  T y = __shfl_up(x, 1, warp_size);
+  return lane_id() == 0 ? first : y;
+}
+
+// Butterfly reduction over the wave (every lane receives the total).
+template <typename T>
+__device__ inline T wave_reduce_all(T x) {
+#pragma unroll
+  for (int offset = 1; offset < warp_size; offset <<= 1)
+    x += __shfl_xor(x, offset, warp_size);
+  return x;
+}
+
+// Lanes of the wave whose 8-bit digit equals the caller's (match_any over
+// eight ballots), restricted to lanes in `active`.
+__device__ inline uint64_t match_digit8(uint32_t digit, uint64_t active) {
+  uint64_t m = active;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    bool bit = (digit >> b) & 1;
+    uint64_t ballot = __ballot(bit);
+    m &= bit ? ballot : ~ballot;
+  }
+  return m;
+}
+
+// Relaxed agent-scope atomics for the look-back state words.  Each word holds
+// its payload and status together, so no ordering with other memory is needed.
+template <typename T>
+__device__ inline T atomic_load_agent(const T* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline void atomic_store_agent(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T atomic_add_agent(T* p, T v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Zero-fill of 32-bit words (used instead of hipMemsetAsync so the
+// schedule stays capturable in a graph and replays correctly).
+template <int NT>
+__global__ __launch_bounds__(NT) void kernel_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+inline hipError_t fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = min_<int64_t>(div_up(n, (int64_t)256), 4096);
+  return launch_kernel("mgpu_fill", kernel_fill_u32<256>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+}  // namespace hip
+}  // namespace mgpu

@@ -1,0 +1,368 @@
+// Device-wide scan and reduce for gfx950.
+//
+// Reference: vk::scan (inc/mgpu/vk/scan.hxx:10-115) is an exclusive,
+// in-place, three-phase scan (upsweep -> recursive scan of tile partials ->
+// downsweep) that moves 12 bytes per 4-byte element.  Here the scan is a
+// single pass with decoupled look-back (Merrill & Garland, NVR-2016-002): each
+// tile publishes its aggregate, then walks back over its predecessors' words
+// 64 at a time (one wave) until it meets an inclusive prefix, so every element
+// is read once and written once (8 bytes per element).
+//
+// Summation order inside a tile is hierarchical as in the reference
+// (thread-serial over 4 consecutive elements, cta_scan.hxx:150; a wave scan,
+// :100-104; the wave totals in order, :107-130; then the carry-in, :157-171).
+// Between tiles the carry is float64 for float32 data; for integers all
+// arithmetic is modulo 2^32.
+#pragma once
+#include "meta.hxx"
+
+namespace mgpu {
+namespace hip {
+
+template <typename T> struct scan_traits;
+template <> struct scan_traits<int32_t> { typedef uint32_t acc_t; typedef uint32_t carry_t; };
+template <> struct scan_traits<uint32_t> { typedef uint32_t acc_t; typedef uint32_t carry_t; };
+template <> struct scan_traits<float> { typedef float acc_t; typedef double carry_t; };
+
+// Look-back status word: the payload and a 2-bit flag (0 = not ready,
+// 1 = tile aggregate, 2 = inclusive prefix) in one 64-bit word.
+enum : uint32_t { status_invalid = 0, status_aggregate = 1, status_inclusive = 2 };
+
+__device__ inline uint64_t pack_status(uint32_t v, uint32_t flag) {
+  return ((uint64_t)v << 32) | flag;
+}
+__device__ inline uint64_t pack_status(double v, uint32_t flag) {
+  return ((uint64_t)__double_as_longlong(v) & ~3ull) | flag;
+}
+template <typename carry_t> __device__ inline carry_t unpack_status(uint64_t w);
+template <> __device__ inline uint32_t unpack_status<uint32_t>(uint64_t w) {
+  return (uint32_t)(w >> 32);
+}
+template <> __device__ inline double unpack_status<double>(uint64_t w) {
+  return __longlong_as_double((long long)(w & ~3ull));
+}
+
+// Exclusive prefix of tile `tile` from the words of its predecessors.  Called
+// by one full wave; every lane returns the same value.
+template <typename carry_t>
+__device__ inline carry_t scan_lookback(const uint64_t* status, int64_t tile) {
+  int lane = lane_id();
+  carry_t excl = 0;
+  int64_t j = tile - 1;
+  while (true) {
+    int64_t idx = j - lane;
+    uint64_t w = idx >= 0 ? atomic_load_agent(status + idx) : (uint64_t)status_inclusive;
+    uint32_t flag = (uint32_t)(w & 3);
+    uint64_t inc_mask = __ballot(flag == status_inclusive);
+    uint64_t bad_mask = __ballot(flag == status_invalid);
+    uint64_t lowest = inc_mask & (~inc_mask + 1);
+    uint64_t range = inc_mask ? (lowest | (lowest - 1)) : ~0ull;
+    if (bad_mask & range) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    carry_t v = ((range >> lane) & 1) ? unpack_status<carry_t>(w) : carry_t(0);
+    excl += wave_reduce_all(v);
+    if (inc_mask) break;
+    j -= warp_size;
+  }
+  return excl;
+}
+
+struct identity_t {
+  template <typename T>
+  __host__ __device__ T operator()(T x) const { return x; }
+};
+
+__device__ inline uint32_t carry_combine(uint32_t carry, uint32_t local) { return carry + local; }
+__device__ inline float carry_combine(double carry, float local) {
+  return (float)(carry + (double)local);
+}
+
+template <int NT, int VT, bool inclusive, typename in_t, typename T, typename xform_t>
+__global__ __launch_bounds__(NT) void kernel_scan_lookback(
+    const in_t* input, T* output, int64_t count, T init, uint64_t* status,
+    uint32_t* counter, int vec_ok, xform_t xform) {
+  typedef scan_traits<T> traits;
+  typedef typename traits::acc_t acc_t;
+  typedef typename traits::carry_t carry_t;
+  constexpr int NW = NT / warp_size;
+  constexpr int NV = NT * VT;
+  constexpr int NR = VT / 4;
+  static_assert(VT % 4 == 0, "VT must be a multiple of 4");
+  static_assert(sizeof(T) == 4, "scan works on 32-bit element types");
+
+  __shared__ uint32_t s_tile;
+  __shared__ acc_t s_wave[NR][NW];
+  __shared__ carry_t s_carry;
+
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int warp = tid / warp_size;
+  if (tid == 0) s_tile = atomic_add_agent(counter, 1u);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * NV;
+  const bool full = count - base >= NV;
+  const bool vec = full && vec_ok;
+
+  acc_t x[VT];
+  if (vec) {
+    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      u32x4 v = __builtin_nontemporal_load((const u32x4*)(input + base) + (r * NT + tid));
+      in_t e0, e1, e2, e3;
+      uint32_t w0 = v.x, w1 = v.y, w2 = v.z, w3 = v.w;
+      std::memcpy(&e0, &w0, 4); std::memcpy(&e1, &w1, 4);
+      std::memcpy(&e2, &w2, 4); std::memcpy(&e3, &w3, 4);
+      x[4 * r + 0] = (acc_t)(T)xform(e0);
+      x[4 * r + 1] = (acc_t)(T)xform(e1);
+      x[4 * r + 2] = (acc_t)(T)xform(e2);
+      x[4 * r + 3] = (acc_t)(T)xform(e3);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int64_t idx = base + (int64_t)(r * NT + tid) * 4 + e;
+        x[4 * r + e] = idx < count ? (acc_t)(T)xform(input[idx]) : acc_t(0);
+      }
+  }
+
+  // Thread-serial inclusive scan of each group of 4, then a wave scan of the
+  // group totals for every row.
+  acc_t w_excl[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    x[4 * r + 1] += x[4 * r + 0];
+    x[4 * r + 2] += x[4 * r + 1];
+    x[4 * r + 3] += x[4 * r + 2];
+    acc_t incl = wave_scan_incl(x[4 * r + 3]);
+    if (lane == warp_size - 1) s_wave[r][warp] = incl;
+    w_excl[r] = wave_shift_up1(incl, acc_t(0));
+  }
+  __syncthreads();
+
+  acc_t run = 0, off[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    acc_t pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      acc_t v = s_wave[r][w];
+      if (w < warp) pre += v;
+      tot += v;
+    }
+    off[r] = run + pre;
+    run += tot;
+  }
+  const acc_t aggregate = run;
+
+  if (warp == 0) {
+    carry_t carry;
+    if (tile == 0) {
+      carry = (carry_t)(acc_t)init;
+    } else {
+      if (lane == 0) atomic_store_agent(status + tile, pack_status((carry_t)aggregate, status_aggregate));
+      carry = scan_lookback<carry_t>(status, tile);
+    }
+    if (lane == 0) {
+      atomic_store_agent(status + tile, pack_status((carry_t)(carry + (carry_t)aggregate), status_inclusive));
+      s_carry = carry;
+    }
+  }
+  __syncthreads();
+  const carry_t carry = s_carry;
+
+  T y[VT];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    acc_t p = off[r] + w_excl[r];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      acc_t local = inclusive ? (acc_t)(p + x[4 * r + e]) : (e == 0 ? p : (acc_t)(p + x[4 * r + e - 1]));
+      y[4 * r + e] = (T)carry_combine(carry, local);
+    }
+  }
+  if (vec) {
+    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      u32x4 v;
+      uint32_t w[4];
+      std::memcpy(w, &y[4 * r], 16);
+      v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
+      __builtin_nontemporal_store(v, (u32x4*)(output + base) + (r * NT + tid));
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int64_t idx = base + (int64_t)(r * NT + tid) * 4 + e;
+        if (idx < count) output[idx] = y[4 * r + e];
+      }
+  }
+}
+
+// transform_scan: scan of xform(input[i]).  input/output may alias (in place).
+template <int NT = 256, int VT = 16, typename in_t, typename T, typename xform_t>
+mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
+                             const in_t* input, T* output, int64_t count,
+                             mgpu_scan_type_t type, T init, xform_t xform) {
+  constexpr int NV = NT * VT;
+  if (count < 0) return MGPU_INVALID_ARG;
+  if (type != MGPU_SCAN_EXCLUSIVE && type != MGPU_SCAN_INCLUSIVE) return MGPU_INVALID_ARG;
+  if (count == 0) return MGPU_OK;
+  if (!input || !output) return MGPU_INVALID_ARG;
+  int64_t num_tiles = div_up(count, (int64_t)NV);
+  if (num_tiles >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
+  size_t need = aux_bytes<uint64_t>(num_tiles) + aux_bytes<uint32_t>(1);
+  if (!aux_data) {
+    aux_size += need;
+    return MGPU_OK;
+  }
+  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  void* p = aux_data;
+  uint64_t* status = advance_pointer<uint64_t>(p, num_tiles);
+  uint32_t* counter = advance_pointer<uint32_t>(p, 1);
+  hipError_t e = fill_u32((uint32_t*)status, (int64_t)(need / 4), 0u, stream);
+  if (e != hipSuccess) return hip_status(e);
+  int vec_ok = sizeof(in_t) == 4 && 0 == ((uintptr_t)input & 15) && 0 == ((uintptr_t)output & 15);
+  if (type == MGPU_SCAN_INCLUSIVE)
+    e = launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, true, in_t, T, xform_t>,
+                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
+                      init, status, counter, vec_ok, xform);
+  else
+    e = launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, false, in_t, T, xform_t>,
+                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
+                      init, status, counter, vec_ok, xform);
+  return hip_status(e);
+}
+
+template <int NT = 256, int VT = 16, typename T>
+mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
+                   T* output, int64_t count, mgpu_scan_type_t type = MGPU_SCAN_EXCLUSIVE,
+                   T init = T()) {
+  return transform_scan<NT, VT>(aux_data, aux_size, stream, input, output, count, type, init,
+                                identity_t());
+}
+
+// In-place exclusive scan with the reference's argument order
+// (inc/mgpu/vk/scan.hxx:10-13).
+template <int NT = 256, int VT = 16, typename T>
+mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, T* data,
+                   int64_t count, T init = T()) {
+  return scan<NT, VT>(aux_data, aux_size, stream, (const T*)data, data, count,
+                      MGPU_SCAN_EXCLUSIVE, init);
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Reduce: per-block partial sums over a grid-stride loop of 128-bit loads,
+// then one block adds the partials in index order, so the result is
+// deterministic for a given count.  float32 sums in float64.
+
+template <typename T> struct reduce_traits;
+template <> struct reduce_traits<int32_t> { typedef uint32_t acc_t; };
+template <> struct reduce_traits<uint32_t> { typedef uint32_t acc_t; };
+template <> struct reduce_traits<float> { typedef double acc_t; };
+
+template <int NT, typename acc_t>
+__device__ inline acc_t block_reduce(acc_t v, acc_t* s_warp) {
+  constexpr int NW = NT / warp_size;
+  v = wave_reduce_all(v);
+  int warp = threadIdx.x / warp_size;
+  if (lane_id() == 0) s_warp[warp] = v;
+  __syncthreads();
+  acc_t total = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) total += s_warp[w];
+  return total;
+}
+
+template <int NT, typename T>
+__global__ __launch_bounds__(NT) void kernel_reduce_partial(
+    const T* input, int64_t count, int64_t head,
+    typename reduce_traits<T>::acc_t* partials) {
+  typedef typename reduce_traits<T>::acc_t acc_t;
+  typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+  __shared__ acc_t s_warp[NT / warp_size];
+  const int tid = threadIdx.x;
+  acc_t sum = 0;
+  const int64_t nvec = (count - head) / 4;
+  const u32x4* vp = (const u32x4*)(input + head);
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  int64_t i = (int64_t)blockIdx.x * NT + tid;
+  auto add4 = [&](u32x4 v) {
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      T t;
+      std::memcpy(&t, &w[e], 4);
+      sum += (acc_t)t;
+    }
+  };
+  for (; i + 3 * stride < nvec; i += 4 * stride) {
+    u32x4 v0 = __builtin_nontemporal_load(vp + i);
+    u32x4 v1 = __builtin_nontemporal_load(vp + i + stride);
+    u32x4 v2 = __builtin_nontemporal_load(vp + i + 2 * stride);
+    u32x4 v3 = __builtin_nontemporal_load(vp + i + 3 * stride);
+    add4(v0); add4(v1); add4(v2); add4(v3);
+  }
+  for (; i < nvec; i += stride) add4(__builtin_nontemporal_load(vp + i));
+  if (blockIdx.x == 0) {
+    if (tid < head) sum += (acc_t)input[tid];
+    int64_t t = head + nvec * 4 + tid;
+    if (tid < 4 && t < count) sum += (acc_t)input[t];
+  }
+  acc_t total = block_reduce<NT>(sum, s_warp);
+  if (tid == 0) partials[blockIdx.x] = total;
+}
+
+template <int NT, typename T>
+__global__ __launch_bounds__(NT) void kernel_reduce_final(
+    const typename reduce_traits<T>::acc_t* partials, int num, T* out, T init) {
+  typedef typename reduce_traits<T>::acc_t acc_t;
+  __shared__ acc_t s_warp[NT / warp_size];
+  acc_t sum = 0;
+  for (int i = threadIdx.x; i < num; i += NT) sum += partials[i];
+  acc_t total = block_reduce<NT>(sum, s_warp);
+  if (threadIdx.x == 0) {
+    if constexpr (std::is_same_v<T, float>)
+      out[0] = (float)((double)init + total);
+    else
+      out[0] = (T)((acc_t)init + total);
+  }
+}
+
+template <int NT = 256, typename T>
+mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
+                     int64_t count, T* out, T init = T()) {
+  typedef typename reduce_traits<T>::acc_t acc_t;
+  if (count < 0) return MGPU_INVALID_ARG;
+  if (!out) return MGPU_INVALID_ARG;
+  if (count > 0 && !input) return MGPU_INVALID_ARG;
+  int64_t blocks = min_<int64_t>(max_<int64_t>(div_up(count, (int64_t)NT * 16), 1), 2048);
+  size_t need = aux_bytes<acc_t>(blocks);
+  if (!aux_data) {
+    aux_size += need;
+    return MGPU_OK;
+  }
+  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  void* p = aux_data;
+  acc_t* partials = advance_pointer<acc_t>(p, blocks);
+  int64_t mis = ((uintptr_t)input & 15) / sizeof(T);
+  int64_t head = mis ? min_<int64_t>((16 / (int64_t)sizeof(T)) - mis, count) : 0;
+  hipError_t e = launch_kernel("mgpu_reduce_partial", kernel_reduce_partial<NT, T>,
+                               dim3((unsigned)blocks), dim3(NT), 0, stream, input, count, head,
+                               partials);
+  if (e != hipSuccess) return hip_status(e);
+  e = launch_kernel("mgpu_reduce_final", kernel_reduce_final<NT, T>, dim3(1), dim3(NT), 0,
+                    stream, (const acc_t*)partials, (int)blocks, out, init);
+  return hip_status(e);
+}
+
+}  // namespace hip
+}  // namespace mgpu

@@ -1,0 +1,303 @@
+"""Python mirror of the mgpu sort / merge / scan call surface over libmgpu_hip.so.
+
+The names and argument meanings follow the reference's `inc/mgpu/vk` host
+templates (mergesort_keys / mergesort_kv / mergesort_indices,
+inc/mgpu/vk/mergesort.hxx:10-103; merge, inc/mgpu/vk/merge.hxx:12-59;
+radix_sort, inc/mgpu/vk/radix.hxx:195-198; scan, inc/mgpu/vk/scan.hxx:10-13).
+Sorts and scans are in place like the reference; merge writes its output.
+Every call goes through the C ABI in include/mgpu_hip.h: a sizing call with a
+NULL aux buffer, then the real call with an aux buffer of that size.  There is
+no CPU fallback: without the built library every entry point raises.
+
+Tensors are device memory (torch is only used for allocation and streams).
+32-bit unsigned keys may be passed as torch.uint32 or as torch.int32 with
+key_type="u32"; 64-bit unsigned keys as torch.uint64 or torch.int64 with
+key_type="u64".
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmgpu_hip.so")
+
+MGPU_OK, MGPU_INVALID_ARG, MGPU_AUX_TOO_SMALL, MGPU_HIP_ERROR, MGPU_UNSUPPORTED = range(5)
+LESS, GREATER = 0, 1
+EXCLUSIVE, INCLUSIVE = 0, 1
+GEN_KINDS = {"u32": 0, "lowentropy": 1, "i32_0_16": 2, "f32_01": 3, "u64": 4, "f32_pm1": 5,
+             "iota": 6}
+KEY_TYPES = ("u32", "i32", "u64", "f32")
+_CTYPE = {"u32": ctypes.c_uint32, "i32": ctypes.c_int32, "u64": ctypes.c_uint64,
+          "f32": ctypes.c_float}
+_DTYPE_KEY = {torch.int32: "i32", torch.float32: "f32", torch.uint32: "u32",
+              torch.uint64: "u64"}
+_KEY_BYTES = {"u32": 4, "i32": 4, "u64": 8, "f32": 4}
+
+
+class MgpuError(RuntimeError):
+    def __init__(self, status, where):
+        self.status = status
+        lib = _lib_or_none()
+        name = lib.mgpu_status_string(status).decode() if lib else str(status)
+        extra = ""
+        if status == MGPU_HIP_ERROR and lib:
+            extra = " (hipError_t %d)" % lib.mgpu_last_hip_error()
+        super().__init__("%s failed: %s%s" % (where, name, extra))
+
+
+_lib = None
+
+
+def _lib_or_none():
+    return _lib
+
+
+def load():
+    """Load libmgpu_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libmgpu_hip.so not built (run python mgpu-shaders_amd/build.py): "
+                          + LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, sz, i64, c_int = ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int64, ctypes.c_int
+    lib.mgpu_status_string.restype = ctypes.c_char_p
+    lib.mgpu_status_string.argtypes = [c_int]
+    lib.mgpu_last_hip_error.restype = c_int
+    lib.mgpu_version.restype = c_int
+    for k in KEY_TYPES:
+        for name, args in [
+            ("mgpu_mergesort_keys_%s", [vp, sz, vp, vp, i64, c_int]),
+            ("mgpu_mergesort_kv_%s_u32", [vp, sz, vp, vp, vp, i64, c_int]),
+            ("mgpu_mergesort_indices_%s", [vp, sz, vp, vp, vp, i64, c_int]),
+            ("mgpu_merge_keys_%s", [vp, sz, vp, vp, i64, vp, i64, vp, c_int]),
+            ("mgpu_merge_kv_%s_u32", [vp, sz, vp, vp, vp, i64, vp, vp, i64, vp, vp, c_int]),
+            ("mgpu_radix_sort_keys_%s", [vp, sz, vp, vp, i64]),
+            ("mgpu_radix_sort_kv_%s_u32", [vp, sz, vp, vp, vp, i64]),
+        ]:
+            f = getattr(lib, name % k)
+            f.restype = c_int
+            f.argtypes = args
+    for t in ("i32", "u32", "f32"):
+        ct = _CTYPE[t]
+        f = getattr(lib, "mgpu_scan_exclusive_%s" % t)
+        f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, ct]
+        f = getattr(lib, "mgpu_scan_%s" % t)
+        f.restype, f.argtypes = c_int, [vp, sz, vp, vp, vp, i64, c_int, ct]
+        f = getattr(lib, "mgpu_reduce_%s" % t)
+        f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, vp, ct]
+    lib.mgpu_generate.restype = c_int
+    lib.mgpu_generate.argtypes = [vp, vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
+    lib.mgpu_timing_enable.restype = None
+    lib.mgpu_timing_enable.argtypes = [c_int]
+    lib.mgpu_timing_collect.restype = c_int
+    lib.mgpu_timing_collect.argtypes = [ctypes.POINTER(ctypes.c_char_p),
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_double), c_int]
+    _lib = lib
+    return lib
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _check(status, where):
+    if status != MGPU_OK:
+        raise MgpuError(status, where)
+
+
+def _key_type(t, key_type):
+    if key_type is not None:
+        if key_type not in KEY_TYPES:
+            raise ValueError("key_type must be one of %s" % (KEY_TYPES,))
+        if t.element_size() != _KEY_BYTES[key_type]:
+            raise ValueError("tensor element size does not match key_type %s" % key_type)
+        return key_type
+    if t.dtype not in _DTYPE_KEY:
+        raise ValueError("cannot infer key type of %s; pass key_type" % t.dtype)
+    return _DTYPE_KEY[t.dtype]
+
+
+def _comp(comp):
+    if comp in ("less", LESS):
+        return LESS
+    if comp in ("greater", GREATER):
+        return GREATER
+    raise ValueError("comp must be 'less' or 'greater'")
+
+
+def _device_check(*ts):
+    for t in ts:
+        if t is not None:
+            if not t.is_cuda:
+                raise ValueError("mgpu_hip operates on device tensors")
+            if not t.is_contiguous():
+                raise ValueError("mgpu_hip needs contiguous tensors")
+
+
+def _run_two_phase(fn, args, where, aux=None, stream=None):
+    """Sizing call (aux=NULL) then the real call; allocates aux unless given."""
+    lib = load()
+    size = ctypes.c_size_t(0)
+    _check(fn(None, ctypes.byref(size), _stream(stream), *args), where + " (sizing)")
+    if aux is None:
+        aux = torch.empty(max(int(size.value), 1), dtype=torch.uint8, device="cuda")
+    cap = ctypes.c_size_t(aux.numel() * aux.element_size())
+    _check(fn(_ptr(aux), ctypes.byref(cap), _stream(stream), *args), where)
+    return aux
+
+
+def aux_size(fn_name, *args, stream=None):
+    """Bytes of aux the named C entry point needs for these arguments."""
+    lib = load()
+    size = ctypes.c_size_t(0)
+    _check(getattr(lib, fn_name)(None, ctypes.byref(size), _stream(stream), *args), fn_name)
+    return int(size.value)
+
+
+# ---------------------------------------------------------------------------
+# Mergesort (inc/mgpu/vk/mergesort.hxx:10-103)
+
+def mergesort_keys(keys, comp="less", key_type=None, aux=None, stream=None):
+    _device_check(keys)
+    k = _key_type(keys, key_type)
+    fn = getattr(load(), "mgpu_mergesort_keys_%s" % k)
+    return _run_two_phase(fn, [_ptr(keys), keys.numel(), _comp(comp)], "mergesort_keys", aux, stream)
+
+
+def mergesort_kv(keys, vals, comp="less", key_type=None, aux=None, stream=None):
+    _device_check(keys, vals)
+    if vals.numel() != keys.numel() or vals.element_size() != 4:
+        raise ValueError("vals must be 32-bit with one per key")
+    k = _key_type(keys, key_type)
+    fn = getattr(load(), "mgpu_mergesort_kv_%s_u32" % k)
+    return _run_two_phase(fn, [_ptr(keys), _ptr(vals), keys.numel(), _comp(comp)],
+                          "mergesort_kv", aux, stream)
+
+
+def mergesort_indices(keys, indices, comp="less", key_type=None, aux=None, stream=None):
+    _device_check(keys, indices)
+    if indices.numel() != keys.numel() or indices.dtype != torch.int32:
+        raise ValueError("indices must be int32 with one per key")
+    k = _key_type(keys, key_type)
+    fn = getattr(load(), "mgpu_mergesort_indices_%s" % k)
+    return _run_two_phase(fn, [_ptr(keys), _ptr(indices), keys.numel(), _comp(comp)],
+                          "mergesort_indices", aux, stream)
+
+
+# ---------------------------------------------------------------------------
+# Merge (inc/mgpu/vk/merge.hxx:12-59)
+
+def merge_keys(a_keys, b_keys, c_keys, comp, key_type=None, aux=None, stream=None):
+    _device_check(a_keys, b_keys, c_keys)
+    if c_keys.numel() != a_keys.numel() + b_keys.numel():
+        raise ValueError("c must hold a + b elements")
+    k = _key_type(a_keys, key_type)
+    fn = getattr(load(), "mgpu_merge_keys_%s" % k)
+    return _run_two_phase(fn, [_ptr(a_keys), a_keys.numel(), _ptr(b_keys), b_keys.numel(),
+                               _ptr(c_keys), _comp(comp)], "merge_keys", aux, stream)
+
+
+def merge(a_keys, a_vals, b_keys, b_vals, c_keys, c_vals, comp, key_type=None, aux=None,
+          stream=None):
+    _device_check(a_keys, a_vals, b_keys, b_vals, c_keys, c_vals)
+    if c_keys.numel() != a_keys.numel() + b_keys.numel():
+        raise ValueError("c must hold a + b elements")
+    k = _key_type(a_keys, key_type)
+    fn = getattr(load(), "mgpu_merge_kv_%s_u32" % k)
+    return _run_two_phase(fn, [_ptr(a_keys), _ptr(a_vals), a_keys.numel(), _ptr(b_keys),
+                               _ptr(b_vals), b_keys.numel(), _ptr(c_keys), _ptr(c_vals),
+                               _comp(comp)], "merge", aux, stream)
+
+
+# ---------------------------------------------------------------------------
+# Radix sort (inc/mgpu/vk/radix.hxx:195-470, plus values)
+
+def radix_sort(keys, vals=None, key_type=None, aux=None, stream=None):
+    _device_check(keys, vals)
+    k = _key_type(keys, key_type)
+    lib = load()
+    if vals is None:
+        fn = getattr(lib, "mgpu_radix_sort_keys_%s" % k)
+        return _run_two_phase(fn, [_ptr(keys), keys.numel()], "radix_sort", aux, stream)
+    if vals.numel() != keys.numel() or vals.element_size() != 4:
+        raise ValueError("vals must be 32-bit with one per key")
+    fn = getattr(lib, "mgpu_radix_sort_kv_%s_u32" % k)
+    return _run_two_phase(fn, [_ptr(keys), _ptr(vals), keys.numel()], "radix_sort", aux, stream)
+
+
+# ---------------------------------------------------------------------------
+# Scan / reduce (inc/mgpu/vk/scan.hxx:10-115)
+
+_SCAN_T = {torch.int32: "i32", torch.uint32: "u32", torch.float32: "f32"}
+
+
+def _scan_type(t, elem_type):
+    if elem_type is not None:
+        return elem_type
+    if t.dtype not in _SCAN_T:
+        raise ValueError("scan supports int32, uint32 and float32")
+    return _SCAN_T[t.dtype]
+
+
+def scan(data, init=0, elem_type=None, aux=None, stream=None):
+    """In-place exclusive plus-scan: data[i] = init + sum(data[:i])."""
+    _device_check(data)
+    t = _scan_type(data, elem_type)
+    fn = getattr(load(), "mgpu_scan_exclusive_%s" % t)
+    return _run_two_phase(fn, [_ptr(data), data.numel(), _CTYPE[t](init)], "scan", aux, stream)
+
+
+def scan_into(input, output, inclusive=False, init=0, elem_type=None, aux=None, stream=None):
+    """Out-of-place (or in-place when output is input) exclusive/inclusive scan."""
+    _device_check(input, output)
+    if output.numel() != input.numel():
+        raise ValueError("output must match input")
+    t = _scan_type(input, elem_type)
+    fn = getattr(load(), "mgpu_scan_%s" % t)
+    return _run_two_phase(fn, [_ptr(input), _ptr(output), input.numel(),
+                               INCLUSIVE if inclusive else EXCLUSIVE, _CTYPE[t](init)],
+                          "scan", aux, stream)
+
+
+def reduce(input, out, init=0, elem_type=None, aux=None, stream=None):
+    """out[0] = init + sum(input) (out is a 1-element device tensor)."""
+    _device_check(input, out)
+    t = _scan_type(input, elem_type)
+    fn = getattr(load(), "mgpu_reduce_%s" % t)
+    return _run_two_phase(fn, [_ptr(input), input.numel(), _ptr(out), _CTYPE[t](init)],
+                          "reduce", aux, stream)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic inputs and kernel timing
+
+def generate(out, kind, seed=0, offset=0, stream=None):
+    """Fill `out` with element (offset + i) of a SplitMix64 generator kind."""
+    _device_check(out)
+    _check(load().mgpu_generate(_stream(stream), _ptr(out), out.numel(), out.element_size(),
+                                GEN_KINDS[kind], seed, offset), "generate")
+    return out
+
+
+def timing_enable(enable=True):
+    load().mgpu_timing_enable(1 if enable else 0)
+
+
+def timing_collect():
+    """{kernel_name: (launches, total_ms)} of launches since timing_enable."""
+    lib = load()
+    n = lib.mgpu_timing_collect(None, None, None, 0)
+    names = (ctypes.c_char_p * max(n, 1))()
+    launches = (ctypes.c_int * max(n, 1))()
+    total = (ctypes.c_double * max(n, 1))()
+    lib.mgpu_timing_collect(names, launches, total, n)
+    return {names[i].decode(): (launches[i], total[i]) for i in range(n)}

@@ -1,0 +1,126 @@
+"""C ABI checks that need no GPU: exports, sizing calls, argument errors."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import mgpu_hip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mgpu_hip.h")
+
+
+def declared_functions():
+    pre = subprocess.run(["gcc", "-E", "-P", HEADER], capture_output=True, text=True, check=True)
+    return sorted(set(re.findall(r"\b(mgpu_[a-z0-9_]+)\s*\(", pre.stdout)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = mgpu_hip.load()
+    names = declared_functions()
+    assert len(names) >= 40
+    nm = subprocess.run(["nm", "-D", "--defined-only", mgpu_hip.LIB_PATH], capture_output=True,
+                        text=True, check=True).stdout
+    exported = set(re.findall(r" T (mgpu_\w+)", nm))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        assert getattr(lib, n) is not None
+
+
+def test_status_strings():
+    lib = mgpu_hip.load()
+    assert lib.mgpu_status_string(0) == b"MGPU_OK"
+    assert lib.mgpu_status_string(2) == b"MGPU_AUX_TOO_SMALL"
+    assert lib.mgpu_version() >= 100
+
+
+def _size(fn, *args):
+    size = ctypes.c_size_t(0)
+    st = fn(None, ctypes.byref(size), None, *args)
+    return st, size.value
+
+
+def r256(x):
+    return (x + 255) // 256 * 256
+
+
+def test_sizing_mergesort_matches_formula():
+    lib = mgpu_hip.load()
+    n = 1 << 20
+    nv = 256 * 15
+    tiles = -(-n // nv)
+    st, sz = _size(lib.mgpu_mergesort_kv_u32_u32, ctypes.c_void_p(1), ctypes.c_void_p(1), n, 0)
+    assert st == 0
+    assert sz == r256(8 * (tiles + 1)) + r256(4 * n) + r256(4 * n)
+    st, sz1 = _size(lib.mgpu_mergesort_keys_u32, ctypes.c_void_p(1), n, 0)
+    assert sz1 == r256(8 * (tiles + 1)) + r256(4 * n)
+    # One tile: no aux bytes (vk/mergesort.hxx:22-23).
+    st, sz = _size(lib.mgpu_mergesort_keys_u32, ctypes.c_void_p(1), 100, 0)
+    assert st == 0 and sz == 0
+
+
+def test_sizing_accumulates():
+    lib = mgpu_hip.load()
+    size = ctypes.c_size_t(1000)
+    st = lib.mgpu_merge_keys_u32(None, ctypes.byref(size), None, ctypes.c_void_p(1), 5000,
+                                 ctypes.c_void_p(1), 5000, ctypes.c_void_p(1), 0)
+    assert st == 0
+    assert size.value == 1000 + r256(8 * (-(-10000 // (256 * 15)) + 1))
+
+
+def test_sizing_radix_and_scan():
+    lib = mgpu_hip.load()
+    n = 1 << 20
+    st, sz = _size(lib.mgpu_radix_sort_kv_u32_u32, ctypes.c_void_p(1), ctypes.c_void_p(1), n)
+    tiles = -(-n // (512 * 16))
+    assert st == 0
+    assert sz == r256(4 * n) * 2 + r256(4 * (4 * 256 + 4 * tiles * 256 + 4))
+    st, sz = _size(lib.mgpu_scan_exclusive_i32, ctypes.c_void_p(1), n, 0)
+    assert st == 0 and sz == r256(8 * (n // 4096)) + 256
+
+
+def test_invalid_arguments():
+    lib = mgpu_hip.load()
+    one = ctypes.c_void_p(1)
+    st, _ = _size(lib.mgpu_mergesort_keys_u32, one, -1, 0)
+    assert st == mgpu_hip.MGPU_INVALID_ARG
+    st, _ = _size(lib.mgpu_mergesort_keys_u32, None, 10, 0)
+    assert st == mgpu_hip.MGPU_INVALID_ARG
+    st, _ = _size(lib.mgpu_mergesort_keys_u32, one, 10, 7)
+    assert st == mgpu_hip.MGPU_INVALID_ARG
+    st, _ = _size(lib.mgpu_radix_sort_keys_u32, one, 1 << 30)
+    assert st == mgpu_hip.MGPU_UNSUPPORTED
+    st, _ = _size(lib.mgpu_scan_i32, one, one, 10, 5, 0)
+    assert st == mgpu_hip.MGPU_INVALID_ARG
+    # aux_size pointer is mandatory.
+    assert lib.mgpu_scan_exclusive_i32(None, None, None, one, 10, 0) == mgpu_hip.MGPU_INVALID_ARG
+
+
+def test_count_zero_is_noop():
+    lib = mgpu_hip.load()
+    for fn, args in [(lib.mgpu_mergesort_keys_f32, (None, 0, 0)),
+                     (lib.mgpu_radix_sort_keys_u64, (None, 0)),
+                     (lib.mgpu_scan_exclusive_f32, (None, 0, 0.0)),
+                     (lib.mgpu_merge_keys_i32, (None, 0, None, 0, None, 0))]:
+        st, sz = _size(fn, *args)
+        assert st == 0 and sz == 0
+
+
+def test_aux_too_small_is_reported_before_any_launch():
+    lib = mgpu_hip.load()
+    one = ctypes.c_void_p(1)
+    cap = ctypes.c_size_t(16)
+    st = lib.mgpu_mergesort_keys_u32(one, ctypes.byref(cap), None, one, 1 << 20, 0)
+    assert st == mgpu_hip.MGPU_AUX_TOO_SMALL
+    cap = ctypes.c_size_t(16)
+    st = lib.mgpu_radix_sort_keys_u32(one, ctypes.byref(cap), None, one, 1 << 20)
+    assert st == mgpu_hip.MGPU_AUX_TOO_SMALL
+
+
+def test_python_surface_rejects_host_tensors():
+    import torch
+    with pytest.raises(ValueError):
+        mgpu_hip.mergesort_keys(torch.zeros(10, dtype=torch.int32))

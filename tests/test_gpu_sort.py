@@ -1,0 +1,220 @@
+"""GPU parity: mergesort, merge and radix sort against the oracle, bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+import mgpu_hip
+import oracle
+from gpu_util import banner, dev_equal, gen_dev, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+KTS = ("u32", "i32", "u64", "f32")
+MS_NV = 256 * 15      # mergesort tile of the C ABI instantiation
+RADIX_NV = 512 * 16   # radix tile (u32/i32/f32)
+SIZES = [1, 2, 7, 64, 100, MS_NV - 1, MS_NV, MS_NV + 1, 2 * MS_NV + 3, 5 * MS_NV - 11,
+         RADIX_NV + 1, 100003, (1 << 20) + 17]
+
+
+def make_keys(kt, n, seed, pattern="random"):
+    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1"}[kt]
+    if pattern == "lowentropy" and kt != "u64" and kt != "f32":
+        kind = "lowentropy"
+    x = oracle.generate(n, kind, seed)
+    if kt == "i32":
+        x = x.view(np.int32)
+    if pattern == "ties":
+        x = x[np.arange(n) % 97] if n else x
+    elif pattern == "equal":
+        x = np.full(n, x[0] if n else 0, dtype=x.dtype)
+    elif pattern == "sorted":
+        x = np.sort(x)
+    elif pattern == "reversed":
+        x = np.sort(x)[::-1].copy()
+    return x
+
+
+def dev_keys(x):
+    return to_dev(x)
+
+
+# ------------------------------------------------------------------ mergesort
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("kt", KTS)
+def test_mergesort_kv(n, kt):
+    banner("mergesort_kv", kt, n)
+    keys = make_keys(kt, n, 1, "ties")
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.mergesort_kv(dk, dv)
+    ek, ev = oracle.stable_sort(keys, vals, kt)
+    np.testing.assert_array_equal(to_host(dk), ek)
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
+@pytest.mark.parametrize("pattern", ["random", "equal", "sorted", "reversed", "lowentropy"])
+@pytest.mark.parametrize("comp", ["less", "greater"])
+def test_mergesort_patterns(pattern, comp):
+    n = 3 * MS_NV + 1001
+    banner("mergesort pattern", pattern, comp)
+    keys = make_keys("u32", n, 2, pattern)
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.mergesort_kv(dk, dv, comp=comp)
+    ek, ev = oracle.stable_sort(keys, vals, "u32", greater=(comp == "greater"))
+    np.testing.assert_array_equal(to_host(dk), ek)
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
+@pytest.mark.parametrize("kt", KTS)
+@pytest.mark.parametrize("comp", ["less", "greater"])
+def test_mergesort_keys(kt, comp):
+    n = 200001
+    banner("mergesort_keys", kt, comp)
+    keys = make_keys(kt, n, 3)
+    dk = dev_keys(keys)
+    mgpu_hip.mergesort_keys(dk, comp=comp)
+    np.testing.assert_array_equal(to_host(dk), oracle.stable_sort(keys, kt=kt, greater=(comp == "greater")))
+
+
+def test_mergesort_indices_gl_sort_known_answer():
+    # examples/gl_sort/gl_sort.cxx:33-42: 10000 floats rand() % 10000 sorted
+    # with indices; gathered == sorted and the output is sorted.
+    banner("gl_sort kat")
+    keys = (oracle.glibc_rand(10000) % 10000).astype(np.float32)
+    dk = dev_keys(keys)
+    di = torch.empty(10000, dtype=torch.int32, device="cuda")
+    mgpu_hip.mergesort_indices(dk, di)
+    k, idx = to_host(dk), to_host(di)
+    np.testing.assert_array_equal(keys[idx], k)
+    assert np.all(k[:-1] <= k[1:])
+    np.testing.assert_array_equal(idx, np.argsort(keys, kind="stable"))
+
+
+def test_mergesort_vk_sort_known_answer():
+    # examples/vk_sort/vk_sort.cxx:15-16: 10000 floats rand() % 100000.
+    banner("vk_sort kat")
+    keys = (oracle.glibc_rand(10000) % 100000).astype(np.float32)
+    dk = dev_keys(keys)
+    mgpu_hip.mergesort_keys(dk)
+    np.testing.assert_array_equal(to_host(dk), np.sort(keys))
+
+
+# ---------------------------------------------------------------------- merge
+
+MERGE_SIZES = [(0, 1), (1, 0), (1, 1), (5, 3000), (3000, 5), (MS_NV, MS_NV), (10007, 20011),
+               (1 << 19, (1 << 19) + 3)]
+
+
+@pytest.mark.parametrize("na,nb", MERGE_SIZES)
+@pytest.mark.parametrize("kt", KTS)
+def test_merge_kv(na, nb, kt):
+    banner("merge_kv", kt, na, nb)
+    a = oracle.stable_sort(make_keys(kt, na, 4, "ties"), kt=kt)
+    b = oracle.stable_sort(make_keys(kt, nb, 5, "ties"), kt=kt)
+    av = np.arange(na, dtype=np.uint32)
+    bv = np.arange(na, na + nb, dtype=np.uint32)
+    da, db, dav, dbv = dev_keys(a), dev_keys(b), to_dev(av), to_dev(bv)
+    dc = torch.empty(na + nb, dtype=da.dtype, device="cuda")
+    dcv = torch.empty(na + nb, dtype=torch.uint32, device="cuda")
+    mgpu_hip.merge(da, dav, db, dbv, dc, dcv, "less")
+    ec, ecv = oracle.merge(a, b, av, bv, kt)
+    np.testing.assert_array_equal(to_host(dc), ec)
+    np.testing.assert_array_equal(to_host(dcv), ecv)
+
+
+@pytest.mark.parametrize("comp", ["less", "greater"])
+def test_merge_keys_all_equal_and_greater(comp):
+    banner("merge_keys", comp)
+    na, nb = 50000, 70001
+    greater = comp == "greater"
+    a = oracle.stable_sort(make_keys("u32", na, 6, "ties"), greater=greater)
+    b = oracle.stable_sort(make_keys("u32", nb, 7, "ties"), greater=greater)
+    da, db = dev_keys(a), dev_keys(b)
+    dc = torch.empty(na + nb, dtype=da.dtype, device="cuda")
+    mgpu_hip.merge_keys(da, db, dc, comp)
+    np.testing.assert_array_equal(to_host(dc), oracle.merge(a, b, greater=greater))
+
+
+def test_merge_vk_merge_known_answer():
+    # examples/vk_merge/vk_merge.cxx:18-27: two sorted arrays of 10000 floats
+    # rand() % 100000 (a first, then b from the same rand() stream).
+    banner("vk_merge kat")
+    r = oracle.glibc_rand(20000) % 100000
+    a = np.sort(r[:10000].astype(np.float32))
+    b = np.sort(r[10000:].astype(np.float32))
+    dc = torch.empty(20000, dtype=torch.float32, device="cuda")
+    mgpu_hip.merge_keys(dev_keys(a), dev_keys(b), dc, "less")
+    np.testing.assert_array_equal(to_host(dc), np.sort(np.concatenate([a, b])))
+
+
+# ---------------------------------------------------------------------- radix
+
+@pytest.mark.parametrize("n", SIZES + [3 * RADIX_NV, 3 * RADIX_NV + 5])
+@pytest.mark.parametrize("kt", KTS)
+def test_radix_kv(n, kt):
+    banner("radix_kv", kt, n)
+    keys = make_keys(kt, n, 8, "ties")
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.radix_sort(dk, dv)
+    ek, ev = oracle.stable_sort_radix(keys, vals, kt)
+    np.testing.assert_array_equal(to_host(dk).view(np.uint8), ek.view(np.uint8))
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
+@pytest.mark.parametrize("pattern", ["random", "equal", "sorted", "reversed", "lowentropy"])
+@pytest.mark.parametrize("kt", ["u32", "u64"])
+def test_radix_patterns(pattern, kt):
+    n = 5 * RADIX_NV + 77
+    banner("radix pattern", pattern, kt)
+    keys = make_keys(kt, n, 9, pattern)
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.radix_sort(dk, dv)
+    ek, ev = oracle.stable_sort_radix(keys, vals, kt)
+    np.testing.assert_array_equal(to_host(dk), ek)
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
+def test_radix_special_floats():
+    banner("radix special floats")
+    base = np.array([np.nan, np.inf, 1.0, 0.0, -0.0, -1.0, -np.inf, 1e-45, -1e-45, 3.4e38,
+                     -3.4e38], dtype=np.float32)
+    neg_nan = np.array([0xFFC00000], dtype=np.uint32).view(np.float32)
+    keys = np.tile(np.concatenate([base, neg_nan]), 3000)
+    vals = np.arange(len(keys), dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.radix_sort(dk, dv)
+    ek, ev = oracle.stable_sort_radix(keys, vals, "f32")
+    np.testing.assert_array_equal(to_host(dk).view(np.uint32), ek.view(np.uint32))
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
+def test_radix_keys_vk_radix2_known_answer():
+    # examples/vk_radix2/vk_radix2.cxx:14-75: 8,388,608 glibc rand() keys,
+    # sorted six times in place, compared with std::sort.
+    banner("vk_radix2 kat")
+    n = 256 * 32 * 32 * 32
+    keys = oracle.glibc_rand(n)
+    dk = dev_keys(keys)
+    aux = None
+    for _ in range(6):
+        aux = mgpu_hip.radix_sort(dk, aux=aux)
+    np.testing.assert_array_equal(to_host(dk), oracle.std_sort_u32(keys))
+
+
+def test_radix_repeat_is_deterministic_and_idempotent():
+    banner("radix repeat")
+    n = (1 << 21) + 5
+    keys = gen_dev(n, "lowentropy", 10)
+    vals = gen_dev(n, "iota")
+    k1, v1 = keys.clone(), vals.clone()
+    mgpu_hip.radix_sort(k1, v1)
+    k2, v2 = keys.clone(), vals.clone()
+    mgpu_hip.radix_sort(k2, v2)
+    assert dev_equal(k1, k2) and dev_equal(v1, v2)
+    k3, v3 = k1.clone(), v1.clone()
+    mgpu_hip.radix_sort(k3, v3)
+    assert dev_equal(k3, k1) and dev_equal(v3, v1)
