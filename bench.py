@@ -2,14 +2,14 @@
 """Benchmark of the sort / merge / scan hot path on one MI355X.
 
 Headline step: one stable key-value LSD radix sort (8-bit digits, 4 digit
-passes + 1 histogram pass) of 2^28 uint32 keys drawn uniform from SplitMix64
+passes of count + scan + scatter) of 2^28 uint32 keys drawn uniform from SplitMix64
 seed 0, with uint32 values set to the original index.  Inputs are generated on
 the device and restored by a device-to-device copy before every step; each
 step's sort is bracketed by HIP events on the launch stream, so the timed
 region starts with the inputs resident in HBM and excludes the restore.
 
 The JSON line also carries:
-  roofline      the dominant kernel (one onesweep digit pass): algorithmic
+  roofline      the dominant kernel (the scatter of one digit pass): algorithmic
                 bytes per launch (16 B/key: read key + value, write key +
                 value) / its average duration from the library's per-launch
                 HIP events, against the 8 TB/s HBM peak; `traffic` is read
@@ -123,7 +123,7 @@ def secondary(torch, mgpu_hip, steps, warmup):
         if algo == "radix":
             aux = mgpu_hip.radix_sort(k, v)
             run = lambda: mgpu_hip.radix_sort(k, v, aux=aux)
-            kname = "mgpu_radix_onesweep"
+            kname = "mgpu_radix_scatter"
         else:
             aux = mgpu_hip.mergesort_kv(k, v)
             run = lambda: mgpu_hip.mergesort_kv(k, v, aux=aux)
@@ -224,10 +224,12 @@ def main():
     aux = mgpu_hip.radix_sort(keys, vals)
     run = lambda: mgpu_hip.radix_sort(keys, vals, aux=aux)
     log("radix kv 2^28: warmup", args.warmup, "steps", args.steps)
+    mgpu_hip.lookback_fallbacks(reset=True)
     ms, wall_ms = timed_steps(torch, args.steps, args.warmup, restore, run)
+    fallbacks = mgpu_hip.lookback_fallbacks(reset=True)
     kt = kernel_times(mgpu_hip, torch, min(args.steps, 5), restore, run)
     log("kernel times", kt)
-    launches, pass_ms = kt["mgpu_radix_onesweep"]
+    launches, pass_ms = kt["mgpu_radix_scatter"]
     pass_bytes = 16 * n
     achieved = pass_bytes / (pass_ms / 1e3) / 1e9
     workload = "radix_sort_kv_u32_2p28_uniform"
@@ -247,16 +249,20 @@ def main():
         "dtype": "u32",
         "data": "synthetic (SplitMix64 seed 0, generated on device)",
         "config": {"workload": "2^28 uint32 keys uniform over [0,2^32), uint32 values = index, "
-                               "stable LSD radix sort, 8-bit digits (4 onesweep passes + 1 "
-                               "histogram pass)",
-                   "n": n, "key": "u32", "value": "u32", "nt": 512, "vt": 16},
-        "roofline": {"bound": "hbm", "kernel": "mgpu_radix_onesweep",
+                               "stable LSD radix sort, 8-bit digits, 4 passes of "
+                               "count + scan + scatter",
+                   "n": n, "key": "u32", "value": "u32", "nt": 512, "vt": 24},
+        "roofline": {"bound": "hbm", "kernel": "mgpu_radix_scatter",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "algorithmic_bytes_per_launch": pass_bytes,
-                     "avg_launch_ms": pass_ms, "launches": launches},
+                     "avg_launch_ms": pass_ms, "launches": launches,
+                     "whole_pass_ms": ms / 4,
+                     "whole_pass_frac": pass_bytes / (ms / 4 / 1e3) / 1e9 / HBM_PEAK_GBS},
         "kernels_ms": {k: round(v[1], 4) for k, v in kt.items()},
+        "lookback": {"spin_limit": mgpu_hip.lookback_spin_limit(),
+                     "fallbacks_in_timed_steps": fallbacks},
     }
     del keys_src, vals_src, keys, vals, aux
     torch.cuda.empty_cache()

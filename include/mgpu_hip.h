@@ -39,7 +39,9 @@ typedef struct ihipStream_t* mgpu_stream_t;
 
 typedef enum mgpu_status_t {
   MGPU_OK = 0,
-  MGPU_INVALID_ARG = 1,   /* NULL pointer with count > 0, negative count, bad enum */
+  MGPU_INVALID_ARG = 1,   /* negative count, bad enum; NULL data pointer with
+                             count > 0 on the call that enqueues (the sizing
+                             call with aux == NULL never reads data pointers) */
   MGPU_AUX_TOO_SMALL = 2, /* *aux_size (capacity) below the requirement          */
   MGPU_HIP_ERROR = 3,     /* a HIP runtime call failed; see mgpu_last_hip_error  */
   MGPU_UNSUPPORTED = 4    /* size beyond a documented limit of the entry point   */
@@ -101,7 +103,7 @@ MGPU_DECL_MERGE(f32, float)
  * value order; signed: MSB flipped; float: -0.0 before +0.0, NaNs last),
  * stable, in place.  Replaces vk::radix_sort (inc/mgpu/vk/radix.hxx:195-470)
  * and extends it with values.  Bit order per inc/mgpu/common/cta_radix.hxx:
- * 6-64.  count < 2^30 (30-bit look-back counters).                          */
+ * 6-64.  count < 2^32 - 8192 (32-bit positions).                           */
 #define MGPU_DECL_RADIX(K, KT)                                                 \
   mgpu_status_t mgpu_radix_sort_keys_##K(void* aux, size_t* aux_size,          \
       mgpu_stream_t stream, KT* keys, int64_t count);                          \
@@ -152,6 +154,18 @@ typedef enum mgpu_gen_kind_t {
 } mgpu_gen_kind_t;
 mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
                             mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset);
+
+/* ------------------------------------------------------------------------ */
+/* Look-back control (no counterpart in the reference, which has no single-
+ * pass kernels).  Scan and radix tiles wait for a predecessor's published
+ * word at most `limit` polls, then compute that predecessor's contribution
+ * from its input, so progress never depends on workgroup dispatch order.
+ * Tests set a tiny limit to force that path; results are identical.
+ * mgpu_lookback_fallbacks returns how often it was taken on the current
+ * device (synchronising the device) and optionally resets the count.       */
+void mgpu_set_lookback_spin_limit(uint32_t limit);
+uint32_t mgpu_lookback_spin_limit(void);
+int64_t mgpu_lookback_fallbacks(int reset);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (HIP events recorded on the launch stream around every

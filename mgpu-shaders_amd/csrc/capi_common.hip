@@ -77,4 +77,25 @@ mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int 
   return hip_status(e);
 }
 
+void mgpu_set_lookback_spin_limit(uint32_t limit) {
+  lookback_settings_t& s = lookback_settings();
+  std::lock_guard<std::mutex> lock(s.mutex);
+  s.spin_limit = limit;
+}
+
+uint32_t mgpu_lookback_spin_limit(void) {
+  lookback_settings_t& s = lookback_settings();
+  std::lock_guard<std::mutex> lock(s.mutex);
+  return s.spin_limit;
+}
+
+int64_t mgpu_lookback_fallbacks(int reset) {
+  lookback_config_t c = lookback_config();
+  if (!c.fallbacks) return -1;
+  uint32_t v = 0;
+  if (hipMemcpy(&v, c.fallbacks, 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  if (reset && hipMemset(c.fallbacks, 0, 4) != hipSuccess) return -1;
+  return (int64_t)v;
+}
+
 }  // extern "C"

@@ -9,15 +9,15 @@ using namespace mgpu::hip;
                                                     mgpu_stream_t stream, KT* keys,         \
                                                     int64_t count) {                        \
     if (!aux_size) return MGPU_INVALID_ARG;                                                 \
-    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt;                     \
-    return radix_sort<NT, VT>(aux, *aux_size, (hipStream_t)stream, keys, count);            \
+    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt, CNT = radix_tuning<KT>::cnt;                     \
+    return radix_sort<NT, VT, CNT>(aux, *aux_size, (hipStream_t)stream, keys, count);            \
   }                                                                                         \
   extern "C" mgpu_status_t mgpu_radix_sort_kv_##K##_u32(void* aux, size_t* aux_size,        \
                                                         mgpu_stream_t stream, KT* keys,     \
                                                         uint32_t* vals, int64_t count) {    \
     if (!aux_size) return MGPU_INVALID_ARG;                                                 \
-    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt;                     \
-    return radix_sort<NT, VT>(aux, *aux_size, (hipStream_t)stream, keys, vals, count);      \
+    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt, CNT = radix_tuning<KT>::cnt;                     \
+    return radix_sort<NT, VT, CNT>(aux, *aux_size, (hipStream_t)stream, keys, vals, count);      \
   }
 
 MGPU_DEF_RADIX(u32, uint32_t)

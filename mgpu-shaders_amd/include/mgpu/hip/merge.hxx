@@ -33,31 +33,18 @@ __device__ inline int merge_path_lds(const key_t* a, int a_count, const key_t* b
   return begin;
 }
 
-// Wave-cooperative merge path in global memory.  All 64 lanes must call it
-// with the same arguments; every lane returns the result.
+// Merge path in global memory, one binary search per thread (the upper levels
+// of every search touch the same few cache lines, which stay in L2).
 template <typename key_t, typename comp_t>
-__device__ inline int64_t merge_path_wave(const key_t* a, int64_t a_count, const key_t* b,
-                                          int64_t b_count, int64_t diag, comp_t comp) {
-  const int lane = lane_id();
+__device__ inline int64_t merge_path_global(const key_t* a, int64_t a_count, const key_t* b,
+                                            int64_t b_count, int64_t diag, comp_t comp) {
   int64_t begin = max_<int64_t>(0, diag - b_count);
   int64_t end = min_<int64_t>(diag, a_count);
   while (begin < end) {
-    int64_t n = end - begin;
-    int64_t step = n <= warp_size ? 1 : div_up(n, (int64_t)warp_size);
-    int64_t s = begin + (int64_t)lane * step;
-    bool valid = s < end;
-    bool pred = false;
-    if (valid) pred = !comp(b[diag - 1 - s], a[s]);
-    uint64_t bal = __ballot(pred);
-    uint64_t vmask = __ballot(valid);
-    int cnt = __popcll(bal);
-    int nvalid = __popcll(vmask);
-    // pred is true exactly for probes below the answer.
-    if (step == 1) return begin + cnt;
-    int64_t nb = cnt == 0 ? begin : begin + (int64_t)(cnt - 1) * step + 1;
-    int64_t ne = cnt < nvalid ? begin + (int64_t)cnt * step : end;
-    begin = nb;
-    end = ne;
+    int64_t mid = (begin + end) >> 1;
+    bool pred = !comp(b[diag - 1 - mid], a[mid]);
+    if (pred) begin = mid + 1;
+    else end = mid;
   }
   return begin;
 }
@@ -86,17 +73,16 @@ __device__ inline void serial_merge(const key_t* keys_shared, int a_begin, int a
   }
 }
 
-// Partition kernel: mp[i] = merge_path(a, b, min(i * spacing, a_count + b_count)).
-// One wave per partition.
+// Partition kernel: mp[i] = merge_path(a, b, min(i * spacing, a_count + b_count))
+// (vk/partition.hxx:9-20).
 template <int NT, typename key_t, typename comp_t>
 __global__ __launch_bounds__(NT) void kernel_merge_partition(
     const key_t* a, int64_t a_count, const key_t* b, int64_t b_count, int64_t spacing,
     int64_t num_partitions, int64_t* mp, comp_t comp) {
-  int64_t index = (int64_t)blockIdx.x * (NT / warp_size) + threadIdx.x / warp_size;
+  int64_t index = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (index >= num_partitions) return;
   int64_t diag = min_<int64_t>(spacing * index, a_count + b_count);
-  int64_t r = merge_path_wave(a, a_count, b, b_count, diag, comp);
-  if (lane_id() == 0) mp[index] = r;
+  mp[index] = merge_path_global(a, a_count, b, b_count, diag, comp);
 }
 
 // LDS footprint of a tile merge.
@@ -202,9 +188,6 @@ mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
   if (a_count < 0 || b_count < 0) return MGPU_INVALID_ARG;
   const int64_t total = a_count + b_count;
   if (total == 0) return MGPU_OK;
-  if ((a_count && !a_keys) || (b_count && !b_keys) || !c_keys) return MGPU_INVALID_ARG;
-  if (has_values && ((a_count && !a_vals) || (b_count && !b_vals) || !c_vals))
-    return MGPU_INVALID_ARG;
   const int64_t num_tiles = div_up(total, (int64_t)NV);
   if (num_tiles >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
   const int64_t num_partitions = num_tiles + 1;
@@ -213,12 +196,15 @@ mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
     aux_size += need;
     return MGPU_OK;
   }
+  if ((a_count && !a_keys) || (b_count && !b_keys) || !c_keys) return MGPU_INVALID_ARG;
+  if (has_values && ((a_count && !a_vals) || (b_count && !b_vals) || !c_vals))
+    return MGPU_INVALID_ARG;
   if (aux_size < need) return MGPU_AUX_TOO_SMALL;
   void* p = aux_data;
   int64_t* mp = advance_pointer<int64_t>(p, num_partitions);
   constexpr int PNT = 256;
   hipError_t e = launch_kernel("mgpu_merge_partition", kernel_merge_partition<PNT, key_t, comp_t>,
-                               dim3((unsigned)div_up(num_partitions, (int64_t)(PNT / warp_size))),
+                               dim3((unsigned)div_up(num_partitions, (int64_t)PNT)),
                                dim3(PNT), 0, stream, a_keys, a_count, b_keys, b_count,
                                (int64_t)NV, num_partitions, mp, comp);
   if (e != hipSuccess) return hip_status(e);

@@ -182,18 +182,17 @@ __global__ __launch_bounds__(NT) void kernel_blocksort(const key_t* keys_in, con
   }
 }
 
-// Partition of one mergesort pass (vk/mergesort.hxx:65-72); one wave each.
+// Partition of one mergesort pass (vk/mergesort.hxx:65-72); one per thread.
 template <int NT, typename key_t, typename comp_t>
 __global__ __launch_bounds__(NT) void kernel_mergesort_partition(
     const key_t* keys, int64_t count, int64_t coop, int64_t spacing, int64_t num_partitions,
     int64_t* mp, comp_t comp) {
-  int64_t index = (int64_t)blockIdx.x * (NT / warp_size) + threadIdx.x / warp_size;
+  int64_t index = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (index >= num_partitions) return;
   merge_range_t range = compute_mergesort_range(count, index, coop, spacing);
   int64_t diag = min_(spacing * index, count) - range.a_begin;
-  int64_t r = merge_path_wave(keys + range.a_begin, range.a_count(), keys + range.b_begin,
-                              range.b_count(), diag, comp);
-  if (lane_id() == 0) mp[index] = r;
+  mp[index] = merge_path_global(keys + range.a_begin, range.a_count(), keys + range.b_begin,
+                                range.b_count(), diag, comp);
 }
 
 // common/kernel_mergesort.hxx:72-121.
@@ -231,7 +230,6 @@ mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream,
   constexpr int NV = NT * VT;
   if (count < 0) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
-  if (!keys || (has_values && !vals)) return MGPU_INVALID_ARG;
   if (count >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
   const int64_t num_ctas = div_up(count, (int64_t)NV);
   const int num_passes = find_log2(num_ctas, true);
@@ -240,6 +238,7 @@ mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream,
     aux_size += need;
     return MGPU_OK;
   }
+  if (!keys || (has_values && !vals)) return MGPU_INVALID_ARG;
   if (aux_size < need) return MGPU_AUX_TOO_SMALL;
   hipError_t e;
   if (num_passes == 0) {
@@ -267,7 +266,7 @@ mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream,
   for (int pass = 0; pass < num_passes; ++pass) {
     int64_t coop = (int64_t)2 << pass;
     e = launch_kernel("mgpu_mergesort_partition", kernel_mergesort_partition<PNT, key_t, comp_t>,
-                      dim3((unsigned)div_up(num_partitions, (int64_t)(PNT / warp_size))),
+                      dim3((unsigned)div_up(num_partitions, (int64_t)PNT)),
                       dim3(PNT), 0, stream, (const key_t*)k_in, count, coop, (int64_t)NV,
                       num_partitions, mp, comp);
     if (e != hipSuccess) return hip_status(e);

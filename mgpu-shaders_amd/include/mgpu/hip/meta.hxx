@@ -177,6 +177,43 @@ inline hipError_t launch_kernel(const char* name, kernel_t kernel, dim3 grid,
 }
 
 ////////////////////////////////////////////////////////////////////////////////
+// Look-back settings.  Tiles are numbered by blockIdx.x (no atomic ticket: a
+// single-address device atomic costs ~11 ns serialised on MI355X, which would
+// dominate a pass).  A tile waits for a predecessor's word at most
+// `spin_limit` polls; after that it computes the predecessor's contribution
+// from the predecessor's input itself, so progress never depends on the order
+// in which workgroups are dispatched.  `fallbacks` (device, per device) counts
+// how often that happened.
+
+struct lookback_config_t {
+  uint32_t spin_limit;
+  uint32_t* fallbacks;
+};
+
+struct lookback_settings_t {
+  std::mutex mutex;
+  uint32_t spin_limit = 512;
+  std::vector<uint32_t*> counters;  // per device
+};
+inline lookback_settings_t& lookback_settings() {
+  static lookback_settings_t s;
+  return s;
+}
+inline lookback_config_t lookback_config() {
+  lookback_settings_t& s = lookback_settings();
+  std::lock_guard<std::mutex> lock(s.mutex);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if ((int)s.counters.size() <= dev) s.counters.resize(dev + 1, nullptr);
+  if (!s.counters[dev]) {
+    uint32_t* p = nullptr;
+    if (hipMalloc(&p, 256) == hipSuccess && hipMemset(p, 0, 256) == hipSuccess)
+      s.counters[dev] = p;
+  }
+  return lookback_config_t{s.spin_limit, s.counters[dev]};
+}
+
+////////////////////////////////////////////////////////////////////////////////
 // wave64 primitives.
 
 __device__ inline int lane_id() { return (int)__lane_id(); }

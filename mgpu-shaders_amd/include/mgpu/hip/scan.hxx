@@ -42,23 +42,65 @@ template <> __device__ inline double unpack_status<double>(uint64_t w) {
   return __longlong_as_double((long long)(w & ~3ull));
 }
 
+struct identity_t {
+  template <typename T>
+  __host__ __device__ T operator()(T x) const { return x; }
+};
+
+// Sum of xform(input[first, first + n)) computed by one wave, in the carry
+// type (used when a predecessor tile has not published its aggregate).
+template <typename carry_t, typename acc_t, typename in_t, typename T, typename xform_t>
+__device__ inline carry_t wave_tile_sum(const in_t* input, int64_t first, int n, xform_t xform) {
+  carry_t sum = 0;
+  for (int e = lane_id(); e < n; e += warp_size) sum += (carry_t)(acc_t)(T)xform(input[first + e]);
+  return wave_reduce_all(sum);
+}
+
 // Exclusive prefix of tile `tile` from the words of its predecessors.  Called
-// by one full wave; every lane returns the same value.
-template <typename carry_t>
-__device__ inline carry_t scan_lookback(const uint64_t* status, int64_t tile) {
-  int lane = lane_id();
+// by one full wave; every lane returns the same value.  Words are read 64 at a
+// time; a window with a not-yet-published word is re-polled up to
+// lb.spin_limit times, then the first missing tile's aggregate is computed
+// from its input.  For an in-place scan the owner of that tile may be writing
+// its output concurrently, but only after its aggregate word is visible, so
+// the word is read again after the data: if it is visible now, the computed
+// sum is discarded.
+template <typename carry_t, typename acc_t, int NV, typename in_t, typename T, typename xform_t>
+__device__ inline carry_t scan_lookback(const uint64_t* status, int64_t tile, carry_t init,
+                                        const in_t* input, xform_t xform,
+                                        lookback_config_t lb) {
+  const int lane = lane_id();
   carry_t excl = 0;
   int64_t j = tile - 1;
+  uint32_t spins = 0;
   while (true) {
-    int64_t idx = j - lane;
-    uint64_t w = idx >= 0 ? atomic_load_agent(status + idx) : (uint64_t)status_inclusive;
-    uint32_t flag = (uint32_t)(w & 3);
-    uint64_t inc_mask = __ballot(flag == status_inclusive);
-    uint64_t bad_mask = __ballot(flag == status_invalid);
-    uint64_t lowest = inc_mask & (~inc_mask + 1);
-    uint64_t range = inc_mask ? (lowest | (lowest - 1)) : ~0ull;
+    const int64_t idx = j - lane;
+    uint64_t w;
+    if (idx >= 0) w = atomic_load_agent(status + idx);
+    else w = pack_status(idx == -1 ? init : carry_t(0), status_inclusive);
+    const uint32_t flag = (uint32_t)(w & 3);
+    const uint64_t inc_mask = __ballot(flag == status_inclusive);
+    const uint64_t bad_mask = __ballot(flag == status_invalid);
+    const uint64_t lowest = inc_mask & (~inc_mask + 1);
+    const uint64_t range = inc_mask ? (lowest | (lowest - 1)) : ~0ull;
     if (bad_mask & range) {
-      __builtin_amdgcn_s_sleep(1);
+      if (++spins <= lb.spin_limit) {
+        __builtin_amdgcn_s_sleep(2);
+        continue;
+      }
+      const int kb = __builtin_ctzll(bad_mask & range);
+      const int64_t jj = j - kb;
+      carry_t agg = wave_tile_sum<carry_t, acc_t, in_t, T>(input, jj * NV, NV, xform);
+      const uint64_t again = atomic_load_agent(status + jj);
+      carry_t v = lane < kb ? unpack_status<carry_t>(w) : carry_t(0);
+      excl += wave_reduce_all(v);
+      spins = 0;
+      if ((again & 3) != status_invalid) {
+        j = jj;  // published meanwhile: continue from its word
+        continue;
+      }
+      excl += agg;
+      if (lane == 0 && lb.fallbacks) atomic_add_agent(lb.fallbacks, 1u);
+      j = jj - 1;
       continue;
     }
     carry_t v = ((range >> lane) & 1) ? unpack_status<carry_t>(w) : carry_t(0);
@@ -69,20 +111,47 @@ __device__ inline carry_t scan_lookback(const uint64_t* status, int64_t tile) {
   return excl;
 }
 
-struct identity_t {
-  template <typename T>
-  __host__ __device__ T operator()(T x) const { return x; }
-};
-
 __device__ inline uint32_t carry_combine(uint32_t carry, uint32_t local) { return carry + local; }
 __device__ inline float carry_combine(double carry, float local) {
   return (float)(carry + (double)local);
 }
 
+// Load tile `tile` (NT*VT elements) as acc_t: 128-bit row-strided loads for a
+// full, aligned tile, guarded scalar loads otherwise.
+template <int NT, int VT, typename acc_t, typename in_t, typename T, typename xform_t>
+__device__ inline void scan_load_tile(const in_t* input, int64_t count, int64_t tile, bool vec_ok,
+                                      xform_t xform, acc_t (&x)[VT]) {
+  constexpr int NV = NT * VT;
+  constexpr int NR = VT / 4;
+  const int tid = threadIdx.x;
+  const int64_t base = tile * NV;
+  if (vec_ok && count - base >= NV) {
+    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      u32x4 v = __builtin_nontemporal_load((const u32x4*)(input + base) + (r * NT + tid));
+      in_t e[4];
+      std::memcpy(e, &v, 16);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[4 * r + q] = (acc_t)(T)xform(e[q]);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int64_t idx = base + (int64_t)(r * NT + tid) * 4 + q;
+        x[4 * r + q] = idx < count ? (acc_t)(T)xform(input[idx]) : acc_t(0);
+      }
+  }
+}
+
+// Single-pass scan: block b scans tile b.  Wave 0 publishes the tile's
+// aggregate and looks back over its predecessors.
 template <int NT, int VT, bool inclusive, typename in_t, typename T, typename xform_t>
 __global__ __launch_bounds__(NT) void kernel_scan_lookback(
-    const in_t* input, T* output, int64_t count, T init, uint64_t* status,
-    uint32_t* counter, int vec_ok, xform_t xform) {
+    const in_t* input, T* output, int64_t count, int64_t num_tiles, T init, uint64_t* status,
+    lookback_config_t lb, int vec_ok, xform_t xform) {
   typedef scan_traits<T> traits;
   typedef typename traits::acc_t acc_t;
   typedef typename traits::carry_t carry_t;
@@ -92,44 +161,17 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
   static_assert(VT % 4 == 0, "VT must be a multiple of 4");
   static_assert(sizeof(T) == 4, "scan works on 32-bit element types");
 
-  __shared__ uint32_t s_tile;
   __shared__ acc_t s_wave[NR][NW];
   __shared__ carry_t s_carry;
 
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int warp = tid / warp_size;
-  if (tid == 0) s_tile = atomic_add_agent(counter, 1u);
-  __syncthreads();
-  const int64_t tile = s_tile;
+  const int64_t tile = blockIdx.x;
   const int64_t base = tile * NV;
-  const bool full = count - base >= NV;
-  const bool vec = full && vec_ok;
 
   acc_t x[VT];
-  if (vec) {
-    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      u32x4 v = __builtin_nontemporal_load((const u32x4*)(input + base) + (r * NT + tid));
-      in_t e0, e1, e2, e3;
-      uint32_t w0 = v.x, w1 = v.y, w2 = v.z, w3 = v.w;
-      std::memcpy(&e0, &w0, 4); std::memcpy(&e1, &w1, 4);
-      std::memcpy(&e2, &w2, 4); std::memcpy(&e3, &w3, 4);
-      x[4 * r + 0] = (acc_t)(T)xform(e0);
-      x[4 * r + 1] = (acc_t)(T)xform(e1);
-      x[4 * r + 2] = (acc_t)(T)xform(e2);
-      x[4 * r + 3] = (acc_t)(T)xform(e3);
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        int64_t idx = base + (int64_t)(r * NT + tid) * 4 + e;
-        x[4 * r + e] = idx < count ? (acc_t)(T)xform(input[idx]) : acc_t(0);
-      }
-  }
+  scan_load_tile<NT, VT, acc_t, in_t, T>(input, count, tile, vec_ok, xform, x);
 
   // Thread-serial inclusive scan of each group of 4, then a wave scan of the
   // group totals for every row.
@@ -165,44 +207,50 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
     if (tile == 0) {
       carry = (carry_t)(acc_t)init;
     } else {
-      if (lane == 0) atomic_store_agent(status + tile, pack_status((carry_t)aggregate, status_aggregate));
-      carry = scan_lookback<carry_t>(status, tile);
+      if (lane == 0) {
+        atomic_store_agent(status + tile, pack_status((carry_t)aggregate, status_aggregate));
+        // The aggregate must be visible before this tile's output (possibly
+        // over its own input) can be: wait for the store to complete.
+        __builtin_amdgcn_s_waitcnt(0);
+      }
+      carry = scan_lookback<carry_t, acc_t, NV, in_t, T>(status, tile, (carry_t)(acc_t)init,
+                                                         input, xform, lb);
     }
     if (lane == 0) {
-      atomic_store_agent(status + tile, pack_status((carry_t)(carry + (carry_t)aggregate), status_inclusive));
+      atomic_store_agent(status + tile,
+                         pack_status((carry_t)(carry + (carry_t)aggregate), status_inclusive));
       s_carry = carry;
     }
   }
   __syncthreads();
   const carry_t carry = s_carry;
 
-  T y[VT];
+  T o[VT];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     acc_t p = off[r] + w_excl[r];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      acc_t local = inclusive ? (acc_t)(p + x[4 * r + e]) : (e == 0 ? p : (acc_t)(p + x[4 * r + e - 1]));
-      y[4 * r + e] = (T)carry_combine(carry, local);
+    for (int q = 0; q < 4; ++q) {
+      acc_t local = inclusive ? (acc_t)(p + x[4 * r + q])
+                              : (q == 0 ? p : (acc_t)(p + x[4 * r + q - 1]));
+      o[4 * r + q] = (T)carry_combine(carry, local);
     }
   }
-  if (vec) {
+  if (vec_ok && count - base >= NV) {
     typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       u32x4 v;
-      uint32_t w[4];
-      std::memcpy(w, &y[4 * r], 16);
-      v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
+      std::memcpy(&v, &o[4 * r], 16);
       __builtin_nontemporal_store(v, (u32x4*)(output + base) + (r * NT + tid));
     }
   } else {
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        int64_t idx = base + (int64_t)(r * NT + tid) * 4 + e;
-        if (idx < count) output[idx] = y[4 * r + e];
+      for (int q = 0; q < 4; ++q) {
+        int64_t idx = base + (int64_t)(r * NT + tid) * 4 + q;
+        if (idx < count) output[idx] = o[4 * r + q];
       }
   }
 }
@@ -216,29 +264,29 @@ mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t strea
   if (count < 0) return MGPU_INVALID_ARG;
   if (type != MGPU_SCAN_EXCLUSIVE && type != MGPU_SCAN_INCLUSIVE) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
-  if (!input || !output) return MGPU_INVALID_ARG;
   int64_t num_tiles = div_up(count, (int64_t)NV);
   if (num_tiles >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
-  size_t need = aux_bytes<uint64_t>(num_tiles) + aux_bytes<uint32_t>(1);
+  size_t need = aux_bytes<uint64_t>(num_tiles);
   if (!aux_data) {
     aux_size += need;
     return MGPU_OK;
   }
+  if (!input || !output) return MGPU_INVALID_ARG;
   if (aux_size < need) return MGPU_AUX_TOO_SMALL;
   void* p = aux_data;
   uint64_t* status = advance_pointer<uint64_t>(p, num_tiles);
-  uint32_t* counter = advance_pointer<uint32_t>(p, 1);
-  hipError_t e = fill_u32((uint32_t*)status, (int64_t)(need / 4), 0u, stream);
+  const lookback_config_t lb = lookback_config();
+  hipError_t e = fill_u32((uint32_t*)status, 2 * num_tiles, 0u, stream);
   if (e != hipSuccess) return hip_status(e);
   int vec_ok = sizeof(in_t) == 4 && 0 == ((uintptr_t)input & 15) && 0 == ((uintptr_t)output & 15);
   if (type == MGPU_SCAN_INCLUSIVE)
     e = launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, true, in_t, T, xform_t>,
                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
-                      init, status, counter, vec_ok, xform);
+                      num_tiles, init, status, lb, vec_ok, xform);
   else
     e = launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, false, in_t, T, xform_t>,
                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
-                      init, status, counter, vec_ok, xform);
+                      num_tiles, init, status, lb, vec_ok, xform);
   return hip_status(e);
 }
 
@@ -342,14 +390,13 @@ mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const
                      int64_t count, T* out, T init = T()) {
   typedef typename reduce_traits<T>::acc_t acc_t;
   if (count < 0) return MGPU_INVALID_ARG;
-  if (!out) return MGPU_INVALID_ARG;
-  if (count > 0 && !input) return MGPU_INVALID_ARG;
   int64_t blocks = min_<int64_t>(max_<int64_t>(div_up(count, (int64_t)NT * 16), 1), 2048);
   size_t need = aux_bytes<acc_t>(blocks);
   if (!aux_data) {
     aux_size += need;
     return MGPU_OK;
   }
+  if (!out || (count > 0 && !input)) return MGPU_INVALID_ARG;
   if (aux_size < need) return MGPU_AUX_TOO_SMALL;
   void* p = aux_data;
   acc_t* partials = advance_pointer<acc_t>(p, blocks);

@@ -90,6 +90,11 @@ def load():
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, vp, ct]
     lib.mgpu_generate.restype = c_int
     lib.mgpu_generate.argtypes = [vp, vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
+    lib.mgpu_set_lookback_spin_limit.restype = None
+    lib.mgpu_set_lookback_spin_limit.argtypes = [ctypes.c_uint32]
+    lib.mgpu_lookback_spin_limit.restype = ctypes.c_uint32
+    lib.mgpu_lookback_fallbacks.restype = ctypes.c_int64
+    lib.mgpu_lookback_fallbacks.argtypes = [c_int]
     lib.mgpu_timing_enable.restype = None
     lib.mgpu_timing_enable.argtypes = [c_int]
     lib.mgpu_timing_collect.restype = c_int
@@ -286,6 +291,20 @@ def generate(out, kind, seed=0, offset=0, stream=None):
     _check(load().mgpu_generate(_stream(stream), _ptr(out), out.numel(), out.element_size(),
                                 GEN_KINDS[kind], seed, offset), "generate")
     return out
+
+
+def set_lookback_spin_limit(limit):
+    """Polls a look-back waits for a predecessor before computing its part itself."""
+    load().mgpu_set_lookback_spin_limit(int(limit))
+
+
+def lookback_spin_limit():
+    return int(load().mgpu_lookback_spin_limit())
+
+
+def lookback_fallbacks(reset=False):
+    """How often a look-back computed a predecessor's contribution (syncs)."""
+    return int(load().mgpu_lookback_fallbacks(1 if reset else 0))
 
 
 def timing_enable(enable=True):

@@ -125,3 +125,35 @@ def test_device_generator_matches_oracle():
     for kind in ["u32", "lowentropy", "i32_0_16", "f32_01", "u64", "f32_pm1", "iota"]:
         t = gen_dev(100003, kind, 9, 77)
         np.testing.assert_array_equal(to_host(t), oracle.generate(100003, kind, 9, 77))
+
+
+@pytest.mark.parametrize("limit", [0, 1])
+@pytest.mark.parametrize("inplace", [True, False])
+def test_scan_forced_lookback_fallback(limit, inplace):
+    """A tiny spin limit makes tiles sum late predecessors from their input
+    (in place: with the re-check of the published word); results unchanged."""
+    banner("scan fallback", limit, inplace)
+    n = 300 * 4096 + 17
+    x = oracle.generate(n, "i32_0_16", 21)
+    f = oracle.generate(n, "f32_01", 22)
+    old = mgpu_hip.lookback_spin_limit()
+    mgpu_hip.lookback_fallbacks(reset=True)
+    try:
+        mgpu_hip.set_lookback_spin_limit(limit)
+        d = to_dev(x)
+        df = to_dev(f)
+        if inplace:
+            mgpu_hip.scan(d)
+            mgpu_hip.scan_into(df, df, inclusive=True)
+            out, outf = d, df
+        else:
+            out, outf = torch.empty_like(d), torch.empty_like(df)
+            mgpu_hip.scan_into(d, out)
+            mgpu_hip.scan_into(df, outf, inclusive=True)
+        torch.cuda.synchronize()
+    finally:
+        mgpu_hip.set_lookback_spin_limit(old)
+    print("fallbacks", mgpu_hip.lookback_fallbacks(), flush=True)
+    np.testing.assert_array_equal(to_host(out).view(np.uint32), oracle.exclusive_scan_u32(x))
+    exact = oracle.inclusive_scan_f64(f)
+    assert np.max(np.abs(to_host(outf).astype(np.float64) - exact) / exact) <= F32_SCAN_RTOL

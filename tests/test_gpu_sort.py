@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 KTS = ("u32", "i32", "u64", "f32")
 MS_NV = 256 * 15      # mergesort tile of the C ABI instantiation
-RADIX_NV = 512 * 16   # radix tile (u32/i32/f32)
+RADIX_NV = 512 * 24   # radix tile (u32/i32/f32)
 SIZES = [1, 2, 7, 64, 100, MS_NV - 1, MS_NV, MS_NV + 1, 2 * MS_NV + 3, 5 * MS_NV - 11,
          RADIX_NV + 1, 100003, (1 << 20) + 17]
 
@@ -218,3 +218,29 @@ def test_radix_repeat_is_deterministic_and_idempotent():
     k3, v3 = k1.clone(), v1.clone()
     mgpu_hip.radix_sort(k3, v3)
     assert dev_equal(k3, k1) and dev_equal(v3, v1)
+
+
+# ------------------------------------------------------ look-back fallback path
+
+@pytest.mark.parametrize("limit", [0, 1])
+@pytest.mark.parametrize("kt", ["u32", "u64", "f32"])
+def test_radix_forced_lookback_fallback(limit, kt):
+    """A tiny spin limit makes tiles count late predecessors from their input;
+    the sort must be unchanged."""
+    banner("radix fallback", kt, limit)
+    n = 40 * RADIX_NV + 123
+    keys = make_keys(kt, n, 12, "ties")
+    vals = np.arange(n, dtype=np.uint32)
+    old = mgpu_hip.lookback_spin_limit()
+    mgpu_hip.lookback_fallbacks(reset=True)
+    try:
+        mgpu_hip.set_lookback_spin_limit(limit)
+        dk, dv = dev_keys(keys), to_dev(vals)
+        mgpu_hip.radix_sort(dk, dv)
+        torch.cuda.synchronize()
+    finally:
+        mgpu_hip.set_lookback_spin_limit(old)
+    print("fallbacks", mgpu_hip.lookback_fallbacks(), flush=True)
+    ek, ev = oracle.stable_sort_radix(keys, vals, kt)
+    np.testing.assert_array_equal(to_host(dk).view(np.uint8), ek.view(np.uint8))
+    np.testing.assert_array_equal(to_host(dv), ev)
