@@ -17,6 +17,8 @@ The JSON line also carries:
                 of the same kernel has been recorded there;
   cpu_baseline  std::stable_sort of (key, index) pairs from the CPU oracle,
                 one core, on a bounded sample of the same distribution;
+  segsort       65536 lognormal segments (~2^25.6 keys): batched segmented
+                sort (eager and graph replay) vs a loop of plain sorts;
   secondary     the other primitives at their workload sizes (mergesort kv
                 2^28, merge 2 x 2^27, scans / reduce 2^29, low-entropy radix).
 
@@ -189,6 +191,87 @@ def secondary(torch, mgpu_hip, steps, warmup):
     return out
 
 
+def bench_segsort(torch, mgpu_hip, steps, warmup):
+    """65536 lognormal segments (median 256, sigma 1.5, clamped to [1, 2^20]),
+    uint32 keys + uint32 values: batched segmented sort (eager and replayed
+    as one graph) against a loop of plain mergesort calls, one per segment,
+    on the same GPU; outputs compared on the device."""
+    import ctypes
+    import numpy as np
+    from mgpu_hip import workloads
+    lens = workloads.lognormal_lengths(65536, 256.0, 1.5, 1, 1 << 20, seed=0)
+    starts = workloads.segment_starts(lens)
+    n = int(lens.sum())
+    dev = "cuda"
+    segs = torch.from_numpy(starts).to(dev)
+    ks = torch.empty(n, dtype=torch.uint32, device=dev)
+    mgpu_hip.generate(ks, "u32", 0)
+    vs = torch.arange(n, dtype=torch.int32, device=dev).view(torch.uint32)
+    k = torch.empty_like(ks)
+    v = torch.empty_like(vs)
+
+    def restore():
+        k.copy_(ks)
+        v.copy_(vs)
+
+    tuning = mgpu_hip.segmented_sort_autotune("u32", True, "less")
+    restore()
+    aux = mgpu_hip.segmented_sort(k, segs, v)
+    run = lambda: mgpu_hip.segmented_sort(k, segs, v, aux=aux)
+    ms_eager, _ = timed_steps(torch, steps, warmup, restore, run)
+    kt = kernel_times(mgpu_hip, torch, 2, restore, run)
+
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        g = mgpu_hip.Graph(stream)
+        with g:
+            mgpu_hip.segmented_sort(k, segs, v, aux=aux, stream=stream)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        ms_graph, _ = timed_steps(torch, steps, warmup, restore, g.launch)
+    torch.cuda.synchronize()
+    batched_k, batched_v = k.clone(), v.clone()
+
+    # Baseline: one plain mergesort call per segment (segments of length <= 1
+    # are skipped), through the C ABI with a pre-sized aux buffer.
+    lib = mgpu_hip.load()
+    fn = lib.mgpu_mergesort_kv_u32_u32
+    big = int(lens.max())
+    aux_loop = mgpu_hip.mergesort_kv(k[:big], v[:big])
+    cap = ctypes.c_size_t(aux_loop.numel())
+    auxp = ctypes.c_void_p(aux_loop.data_ptr())
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    kb, vb = k.data_ptr(), v.data_ptr()
+    work = [(ctypes.c_void_p(kb + 4 * int(s)), ctypes.c_void_p(vb + 4 * int(s)), int(l))
+            for s, l in zip(starts, lens) if l > 1]
+    restore()
+    for kp, vp, l in work[:200]:
+        fn(auxp, ctypes.byref(cap), sp, kp, vp, l, 0)
+    restore()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for kp, vp, l in work:
+        st = fn(auxp, ctypes.byref(cap), sp, kp, vp, l, 0)
+        if st:
+            raise RuntimeError("plain mergesort failed: %d" % st)
+    torch.cuda.synchronize()
+    ms_loop = (time.perf_counter() - t0) * 1e3
+    same = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
+                torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
+    shapes = mgpu_hip.segsort_shapes()
+    out = {"segments": 65536, "keys": n, "log2_keys": float(np.log2(n)),
+           "ms_batched_eager": ms_eager, "ms_batched_graph": ms_graph,
+           "keys_per_s_graph": n / (ms_graph / 1e3),
+           "ms_loop_of_plain_sorts": ms_loop, "plain_sort_calls": len(work),
+           "speedup_vs_loop": ms_loop / ms_graph, "outputs_identical": same,
+           "tuning": {"cap": tuning["cap"], "shape": [list(shapes[s]) for s in tuning["shape"]]},
+           "kernels_ms": {k2: round(v2[1], 4) for k2, v2 in kt.items()}}
+    log("segsort", {k2: out[k2] for k2 in ("ms_batched_graph", "ms_loop_of_plain_sorts",
+                                           "speedup_vs_loop", "outputs_identical")})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,6 +350,8 @@ def main():
     del keys_src, vals_src, keys, vals, aux
     torch.cuda.empty_cache()
     if not args.no_secondary:
+        result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
+        torch.cuda.empty_cache()
         result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
         torch.cuda.empty_cache()
     if not args.no_cpu:

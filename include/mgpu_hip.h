@@ -138,6 +138,56 @@ MGPU_DECL_SCAN(u32, uint32_t)
 MGPU_DECL_SCAN(f32, float)
 
 /* ------------------------------------------------------------------------ */
+/* Batched segmented sort (no counterpart in the reference, which has no
+ * segmented primitives; the call shape follows the mgpu family's
+ * segmented_sort): one stable sort per segment of keys[0..count), where
+ * segments[i] (device, int32, ascending, segments[0] = 0) is the first
+ * element of segment i and the last segment ends at count.  Segments are
+ * bucketed by length and sorted by persistent blocksort kernels (shape per
+ * bucket from the tuning table) or, beyond the largest bucket, by a
+ * segmented LSD radix sort.  No host synchronisation: the call can be
+ * captured in a graph.  count < 2^31.
+ * Tuning: `mgpu_segsort_tuning_t` as returned by mgpu_segmented_sort_autotune
+ * (measured on this device) or NULL for the library's current table (the
+ * default until an autotune of the same key/value/comparator runs).          */
+#define MGPU_SEGSORT_MAX_BUCKETS 6
+typedef struct mgpu_segsort_tuning_t {
+  int num_buckets;
+  int cap[MGPU_SEGSORT_MAX_BUCKETS];   /* bucket b holds lengths <= cap[b]    */
+  int shape[MGPU_SEGSORT_MAX_BUCKETS]; /* blocksort shape index of bucket b    */
+} mgpu_segsort_tuning_t;
+/* Blocksort shapes: index -> (nt, vt); returns the number of shapes.      */
+int mgpu_segsort_shapes(int* nt, int* vt, int capacity);
+#define MGPU_DECL_SEGSORT(K, KT)                                               \
+  mgpu_status_t mgpu_segmented_sort_keys_##K(void* aux, size_t* aux_size,      \
+      mgpu_stream_t stream, KT* keys, int64_t count, const int32_t* segments,  \
+      int64_t num_segments, mgpu_comp_t comp, const mgpu_segsort_tuning_t* t); \
+  mgpu_status_t mgpu_segmented_sort_kv_##K##_u32(void* aux, size_t* aux_size,  \
+      mgpu_stream_t stream, KT* keys, uint32_t* vals, int64_t count,           \
+      const int32_t* segments, int64_t num_segments, mgpu_comp_t comp,         \
+      const mgpu_segsort_tuning_t* t);                                          \
+  mgpu_status_t mgpu_segmented_sort_autotune_##K(mgpu_stream_t stream,          \
+      int has_values, mgpu_comp_t comp, mgpu_segsort_tuning_t* out,            \
+      double* ms_table);
+MGPU_DECL_SEGSORT(u32, uint32_t)
+MGPU_DECL_SEGSORT(i32, int32_t)
+MGPU_DECL_SEGSORT(u64, uint64_t)
+MGPU_DECL_SEGSORT(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* Graph capture of any sequence of library calls on one stream (no
+ * counterpart in the reference, which records command buffers,
+ * inc/mgpu/vk/context.hxx:93-108).  Between begin and end, calls on `stream`
+ * are recorded, not run; mgpu_graph_launch replays them with the same
+ * arguments and buffers.  The library clears its look-back state with
+ * kernels, never hipMemsetAsync, so replays are exact.                      */
+typedef struct mgpu_graph_st* mgpu_graph_t;
+mgpu_status_t mgpu_capture_begin(mgpu_stream_t stream);
+mgpu_status_t mgpu_capture_end(mgpu_stream_t stream, mgpu_graph_t* graph);
+mgpu_status_t mgpu_graph_launch(mgpu_graph_t graph, mgpu_stream_t stream);
+void mgpu_graph_destroy(mgpu_graph_t graph);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic inputs (no counterpart in the reference, whose examples fill
  * host buffers with rand(), e.g. examples/vk_radix2/vk_radix2.cxx:25-27).
  * Counter-based SplitMix64: element i of `kind` for `seed`, written for

@@ -3,6 +3,7 @@
 
 #include "mgpu/hip/generate.hxx"
 #include "mgpu/hip/meta.hxx"
+#include "mgpu/hip/segsort.hxx"
 
 using namespace mgpu::hip;
 
@@ -87,6 +88,54 @@ uint32_t mgpu_lookback_spin_limit(void) {
   lookback_settings_t& s = lookback_settings();
   std::lock_guard<std::mutex> lock(s.mutex);
   return s.spin_limit;
+}
+
+int mgpu_segsort_shapes(int* nt, int* vt, int capacity) {
+  for (int i = 0; i < segsort_num_shapes && i < capacity; ++i) {
+    if (nt) nt[i] = segsort_shapes[i].nt;
+    if (vt) vt[i] = segsort_shapes[i].vt;
+  }
+  return segsort_num_shapes;
+}
+
+struct mgpu_graph_st {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+mgpu_status_t mgpu_capture_begin(mgpu_stream_t stream) {
+  if (!stream) return MGPU_INVALID_ARG;  // the legacy default stream cannot capture
+  (void)lookback_config();                // device-side state exists before capture
+  timing_registry().enabled = false;
+  return hip_status(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+}
+
+mgpu_status_t mgpu_capture_end(mgpu_stream_t stream, mgpu_graph_t* graph) {
+  if (!stream || !graph) return MGPU_INVALID_ARG;
+  *graph = nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+  if (e != hipSuccess) return hip_status(e);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_status(e);
+  }
+  *graph = new mgpu_graph_st{g, x};
+  return MGPU_OK;
+}
+
+mgpu_status_t mgpu_graph_launch(mgpu_graph_t graph, mgpu_stream_t stream) {
+  if (!graph) return MGPU_INVALID_ARG;
+  return hip_status(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+}
+
+void mgpu_graph_destroy(mgpu_graph_t graph) {
+  if (!graph) return;
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  delete graph;
 }
 
 int64_t mgpu_lookback_fallbacks(int reset) {

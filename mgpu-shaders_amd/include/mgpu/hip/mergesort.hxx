@@ -110,30 +110,28 @@ __device__ inline void block_merge_pass(key_t (&k)[VT], val_t (&v)[VT], int tid,
   __syncthreads();
 }
 
-// common/kernel_mergesort.hxx:12-67.
+// Sort tile_count (<= NT*VT) keys (and values) of one tile in LDS and write
+// them out: the body of the blocksort kernel (common/kernel_mergesort.hxx:
+// 12-67), shared with the segmented sort.  With sort_indices the values are
+// index_base + position (common/kernel_mergesort.hxx:43-49).  keys_in may
+// equal keys_out (the tile is fully read before it is written).
 template <int NT, int VT, bool sort_indices, typename key_t, typename val_t, typename comp_t>
-__global__ __launch_bounds__(NT) void kernel_blocksort(const key_t* keys_in, const val_t* vals_in,
-                                                       key_t* keys_out, val_t* vals_out,
-                                                       int64_t count, comp_t comp) {
+__device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in, key_t* keys_out,
+                                     val_t* vals_out, int tile_count, int64_t index_base,
+                                     comp_t comp, blocksort_storage_t<NT, VT, key_t, val_t>& s) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   constexpr int NV = NT * VT;
-  static_assert(is_pow2(NT), "NT must be a power of two");
-  static_assert(VT < 32, "VT must fit the out-of-range mask");
-  __shared__ blocksort_storage_t<NT, VT, key_t, val_t> s;
   const int tid = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * NV;
-  const int tile_count = (int)min_<int64_t>(NV, count - base);
-
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     int j = i * NT + tid;
-    if (j < tile_count) s.keys[j] = keys_in[base + j];
+    if (j < tile_count) s.keys[j] = keys_in[j];
   }
   if constexpr (has_values && !sort_indices) {
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
       int j = i * NT + tid;
-      if (j < tile_count) s.vals[j] = vals_in[base + j];
+      if (j < tile_count) s.vals[j] = vals_in[j];
     }
   }
   __syncthreads();
@@ -143,7 +141,7 @@ __global__ __launch_bounds__(NT) void kernel_blocksort(const key_t* keys_in, con
   for (int i = 0; i < VT; ++i) k[i] = s.keys[VT * tid + i];
   if constexpr (sort_indices) {
 #pragma unroll
-    for (int i = 0; i < VT; ++i) v[i] = (val_t)(base + VT * tid + i);
+    for (int i = 0; i < VT; ++i) v[i] = (val_t)(index_base + VT * tid + i);
   } else if constexpr (has_values) {
 #pragma unroll
     for (int i = 0; i < VT; ++i) v[i] = s.vals[VT * tid + i];
@@ -158,8 +156,10 @@ __global__ __launch_bounds__(NT) void kernel_blocksort(const key_t* keys_in, con
   }
   odd_even_sort<VT, has_values>(k, v, oor, comp);
 #pragma unroll 1
-  for (int pass = 0; (1 << pass) < NT; ++pass)
+  for (int pass = 0; (1 << pass) < NT; ++pass) {
+    if ((VT << pass) >= tile_count) break;  // one sorted run covers the tile
     block_merge_pass<NT, VT, has_values>(k, v, tid, tile_count, pass, comp, s);
+  }
 
 #pragma unroll
   for (int i = 0; i < VT; ++i) s.keys[VT * tid + i] = k[i];
@@ -171,15 +171,31 @@ __global__ __launch_bounds__(NT) void kernel_blocksort(const key_t* keys_in, con
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     int j = i * NT + tid;
-    if (j < tile_count) keys_out[base + j] = s.keys[j];
+    if (j < tile_count) keys_out[j] = s.keys[j];
   }
   if constexpr (has_values) {
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
       int j = i * NT + tid;
-      if (j < tile_count) vals_out[base + j] = s.vals[j];
+      if (j < tile_count) vals_out[j] = s.vals[j];
     }
   }
+}
+
+// common/kernel_mergesort.hxx:12-67.
+template <int NT, int VT, bool sort_indices, typename key_t, typename val_t, typename comp_t>
+__global__ __launch_bounds__(NT) void kernel_blocksort(const key_t* keys_in, const val_t* vals_in,
+                                                       key_t* keys_out, val_t* vals_out,
+                                                       int64_t count, comp_t comp) {
+  constexpr int NV = NT * VT;
+  static_assert(is_pow2(NT), "NT must be a power of two");
+  static_assert(VT < 32, "VT must fit the out-of-range mask");
+  __shared__ blocksort_storage_t<NT, VT, key_t, val_t> s;
+  const int64_t base = (int64_t)blockIdx.x * NV;
+  const int tile_count = (int)min_<int64_t>(NV, count - base);
+  cta_blocksort<NT, VT, sort_indices>(keys_in + base, vals_in ? vals_in + base : vals_in,
+                                      keys_out + base, vals_out ? vals_out + base : vals_out,
+                                      tile_count, base, comp, s);
 }
 
 // Partition of one mergesort pass (vk/mergesort.hxx:65-72); one per thread.

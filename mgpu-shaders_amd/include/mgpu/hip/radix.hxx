@@ -45,30 +45,33 @@ __device__ inline int64_t xcd_tile(int64_t b, int64_t num_tiles) {
   return x * q + min_(x, r) + k;
 }
 
-// Per-tile counts of one digit, digit-major: counts[d * num_tiles + tile].
-// A tile is NV keys (the scatter's tile); CNT threads read it with 128-bit
-// loads (order is irrelevant for counting).  16 sub-histograms (by lane
-// mod 16) keep same-address LDS atomics at most 4 deep even when every lane
-// holds the same digit.
-template <int CNT, int NV, typename key_t>
-__global__ __launch_bounds__(CNT) void kernel_radix_count(const key_t* keys, int64_t count,
-                                                          int shift, uint32_t* counts,
-                                                          int64_t num_tiles) {
-  typedef typename radix_permute_t<key_t>::unsigned_type bits_t;
+// Key -> sort bits.  radix_bits_t is radix_permute_t's order (ascending);
+// the segmented sort also uses descending bits and a float map that ties
+// -0.0 with +0.0, so its order matches comparator semantics.
+template <typename key_t>
+struct radix_bits_t {
+  typedef typename radix_permute_t<key_t>::unsigned_type type;
+  __device__ type operator()(key_t k) const { return radix_permute_t<key_t>::to_radix_bits(k); }
+};
+
+// Counts of one digit over tile_count keys: out[d * stride] = count of d.
+// CNT threads read the tile with 128-bit loads when it is full and aligned
+// (order is irrelevant for counting).  16 sub-histograms (by lane mod 16)
+// keep same-address LDS atomics at most 4 deep even when every lane holds the
+// same digit.
+template <int CNT, int NV, typename key_t, typename xbits_t>
+__device__ inline void cta_radix_count(const key_t* keys, int tile_count, int shift, xbits_t xb,
+                                       uint32_t* out, int64_t stride, uint32_t* h) {
   constexpr int COLS = 16;
   constexpr int KPV = 16 / (int)sizeof(key_t);   // keys per 128-bit vector
   constexpr int NVEC = NV / KPV;                 // vectors per tile
   static_assert(NV % (KPV * CNT) == 0, "tile must split into 128-bit loads");
-  __shared__ uint32_t h[radix_num_bins * COLS];
   const int tid = threadIdx.x;
   for (int i = tid; i < radix_num_bins * COLS; i += CNT) h[i] = 0;
   __syncthreads();
-  const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
-  const int64_t base = tile * NV;
-  const int tile_count = (int)min_<int64_t>(NV, count - base);
   const int col = lane_id() % COLS;
   auto add = [&](key_t key) {
-    uint32_t d = (uint32_t)(radix_permute_t<key_t>::to_radix_bits(key) >> shift) & 255u;
+    uint32_t d = (uint32_t)(xb(key) >> shift) & 255u;
     atomicAdd(&h[d * COLS + col], 1u);
   };
   if (tile_count == NV && 0 == ((uintptr_t)keys & 15)) {
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(CNT) void kernel_radix_count(const key_t* keys, int
     u32x4 v[NVEC / CNT];
 #pragma unroll
     for (int i = 0; i < NVEC / CNT; ++i)
-      v[i] = __builtin_nontemporal_load((const u32x4*)(keys + base) + i * CNT + tid);
+      v[i] = __builtin_nontemporal_load((const u32x4*)keys + i * CNT + tid);
 #pragma unroll
     for (int i = 0; i < NVEC / CNT; ++i) {
       key_t kk[KPV];
@@ -85,73 +88,75 @@ __global__ __launch_bounds__(CNT) void kernel_radix_count(const key_t* keys, int
       for (int e = 0; e < KPV; ++e) add(kk[e]);
     }
   } else {
-    for (int j = tid; j < tile_count; j += CNT) add(keys[base + j]);
+    for (int j = tid; j < tile_count; j += CNT) add(keys[j]);
   }
   __syncthreads();
   for (int d = tid; d < radix_num_bins; d += CNT) {
     uint32_t sum = 0;
 #pragma unroll
     for (int c = 0; c < COLS; ++c) sum += h[d * COLS + c];
-    counts[d * num_tiles + tile] = sum;
+    out[d * stride] = sum;
   }
+}
+
+// Per-tile counts of one digit, digit-major: counts[d * num_tiles + tile].
+template <int CNT, int NV, typename key_t>
+__global__ __launch_bounds__(CNT) void kernel_radix_count(const key_t* keys, int64_t count,
+                                                          int shift, uint32_t* counts,
+                                                          int64_t num_tiles) {
+  __shared__ uint32_t h[radix_num_bins * 16];
+  const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
+  const int64_t base = tile * NV;
+  const int tile_count = (int)min_<int64_t>(NV, count - base);
+  cta_radix_count<CNT, NV>(keys + base, tile_count, shift, radix_bits_t<key_t>(),
+                           counts + tile, num_tiles, h);
 }
 
 template <int NT, int VT, typename key_t>
 struct radix_scatter_storage_t {
-  typedef typename radix_permute_t<key_t>::unsigned_type bits_t;
   static constexpr int NW = NT / warp_size;
   union {
-    bits_t keys[NT * VT];                  // tile reordered by digit (then values)
+    key_t keys[NT * VT];                   // tile reordered by digit (then values)
     uint64_t match[NW][radix_num_bins];    // per-wave lane masks of each digit (ranking)
   };
   uint32_t hist[NW][radix_num_bins];       // per-wave counts -> wave prefixes
   uint32_t local_off[radix_num_bins];      // tile-local exclusive digit offsets
-  uint32_t global_off[radix_num_bins];     // global position of local slot 0 per digit
+  uint32_t global_off[radix_num_bins];     // output position of local slot 0 per digit
   uint32_t scan_tot[NW];
 };
 
-// Scatter of one digit pass.  offsets[d * num_tiles + t] is the global
-// position of tile t's first key with digit d.
-template <int NT, int VT, typename key_t, bool has_values>
-__global__ __launch_bounds__(NT) void kernel_radix_scatter(
-    const key_t* keys_in, key_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-    int64_t count, int shift, const uint32_t* offsets, int64_t num_tiles) {
-  typedef radix_permute_t<key_t> perm_t;
-  typedef typename perm_t::unsigned_type bits_t;
+// Stable scatter of one tile by one digit.  Thread d < 256 passes `goff`,
+// the output position of the tile's first key with digit d (relative to
+// out_base); positions are checked against out_limit.
+template <int NT, int VT, typename key_t, bool has_values, typename xbits_t>
+__device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
+                                         key_t* keys_out, uint32_t* vals_out, int tile_count,
+                                         int shift, xbits_t xb, uint32_t goff, int64_t out_limit,
+                                         radix_scatter_storage_t<NT, VT, key_t>& s) {
   constexpr int NW = NT / warp_size;
-  constexpr int NV = NT * VT;
   static_assert(NT >= radix_num_bins && NT % radix_num_bins == 0, "NT must cover the bins");
-  static_assert(sizeof(bits_t) * NT * VT >= sizeof(uint64_t) * NW * radix_num_bins,
+  static_assert(sizeof(key_t) * NT * VT >= sizeof(uint64_t) * NW * radix_num_bins,
                 "match table must fit in the key buffer");
-  __shared__ radix_scatter_storage_t<NT, VT, key_t> s;
-
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int warp = tid / warp_size;
-  const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
-  const int64_t base = tile * NV;
-  const int tile_count = (int)min_<int64_t>(NV, count - base);
-
-  // Global offset of this tile's digit run (issued first: independent).
-  uint32_t goff = 0;
-  if (tid < radix_num_bins) goff = offsets[tid * num_tiles + tile];
 
   // Each wave owns a contiguous chunk of 64*VT keys; slot i holds keys
   // [chunk + 64 i, chunk + 64 i + 64) in lane order, so wave-local rank order
   // (slot, lane) is the input order.
   const int chunk = warp * warp_size * VT;
-  bits_t k[VT];
+  key_t k[VT];
   uint32_t v[VT];
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     int j = chunk + i * warp_size + lane;
-    k[i] = j < tile_count ? perm_t::to_radix_bits(keys_in[base + j]) : ~(bits_t)0;
+    if (j < tile_count) k[i] = keys_in[j];
   }
   if constexpr (has_values) {
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
       int j = chunk + i * warp_size + lane;
-      v[i] = j < tile_count ? vals_in[base + j] : 0u;
+      v[i] = j < tile_count ? vals_in[j] : 0u;
     }
   }
   // Wave-private counters and match table (LDS operations of one wave are
@@ -171,9 +176,9 @@ __global__ __launch_bounds__(NT) void kernel_radix_scatter(
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     const int j = chunk + i * warp_size + lane;
-    const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
     uint32_t rank = 0;
     if (j < tile_count) {
+      const uint32_t d = (uint32_t)(xb(k[i]) >> shift) & 255u;
       const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
       const uint64_t same = __ballot(d == d0);
       uint64_t m = same;
@@ -222,7 +227,7 @@ __global__ __launch_bounds__(NT) void kernel_radix_scatter(
   for (int i = 0; i < VT; ++i) {
     const int jj = chunk + i * warp_size + lane;
     if (jj < tile_count) {
-      const uint32_t d = (uint32_t)(k[i] >> shift) & 255u;
+      const uint32_t d = (uint32_t)(xb(k[i]) >> shift) & 255u;
       const uint32_t pos = s.local_off[d] + s.hist[warp][d] + r[i];
       s.keys[pos] = k[i];
       r[i] = pos;
@@ -236,10 +241,10 @@ __global__ __launch_bounds__(NT) void kernel_radix_scatter(
     const int p = i * NT + tid;
     dst[i] = 0xFFFFFFFFu;
     if (p < tile_count) {
-      bits_t b = s.keys[p];
-      const uint32_t d = (uint32_t)(b >> shift) & 255u;
+      key_t key = s.keys[p];
+      const uint32_t d = (uint32_t)(xb(key) >> shift) & 255u;
       dst[i] = s.global_off[d] + (uint32_t)p;
-      if (dst[i] < count) keys_out[dst[i]] = perm_t::from_radix_bits(b);
+      if (dst[i] < out_limit) keys_out[dst[i]] = key;
     }
   }
   if constexpr (has_values) {
@@ -254,9 +259,28 @@ __global__ __launch_bounds__(NT) void kernel_radix_scatter(
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
       const int p = i * NT + tid;
-      if (p < tile_count && dst[i] < count) vals_out[dst[i]] = s_vals[p];
+      if (p < tile_count && dst[i] < out_limit) vals_out[dst[i]] = s_vals[p];
     }
   }
+}
+
+// Scatter of one digit pass.  offsets[d * num_tiles + t] is the global
+// position of tile t's first key with digit d.
+template <int NT, int VT, typename key_t, bool has_values>
+__global__ __launch_bounds__(NT) void kernel_radix_scatter(
+    const key_t* keys_in, key_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+    int64_t count, int shift, const uint32_t* offsets, int64_t num_tiles) {
+  constexpr int NV = NT * VT;
+  __shared__ radix_scatter_storage_t<NT, VT, key_t> s;
+  const int tid = threadIdx.x;
+  const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
+  const int64_t base = tile * NV;
+  const int tile_count = (int)min_<int64_t>(NV, count - base);
+  uint32_t goff = 0;
+  if (tid < radix_num_bins) goff = offsets[tid * num_tiles + tile];
+  cta_radix_scatter<NT, VT, key_t, has_values>(keys_in + base, has_values ? vals_in + base : vals_in,
+                                               keys_out, vals_out, tile_count, shift,
+                                               radix_bits_t<key_t>(), goff, count, s);
 }
 
 template <int NT, int VT, typename key_t, typename val_t>

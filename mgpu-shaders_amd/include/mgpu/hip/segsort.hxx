@@ -1,0 +1,476 @@
+// Batched segmented sort for gfx950: thousands of uneven segments per call.
+//
+// No counterpart in the reference (it has no segmented primitives); the call
+// shape follows the mgpu family's segmented_sort: keys (+ values) of `count`
+// elements, `segments[i]` = first element of segment i (segments[0] = 0, the
+// last segment ends at count), one stable sort per segment by `comp`.
+//
+// Every step is a device kernel whose sizes come from device memory, so the
+// whole call is one stream-ordered sequence with no host synchronisation and
+// can be captured in a graph:
+//   1. classify: segment i goes to the first size bucket whose capacity holds
+//      it (blocksort shapes NT x VT, chosen per bucket by the autotuner), to
+//      the `large` bucket beyond the largest capacity, nowhere if len <= 1;
+//      flags[b * S + i] = 1 for that bucket;
+//   2. an exclusive scan of the flags gives each segment's slot in one list
+//      per bucket (segments in index order), and the bucket bounds;
+//   3. per bucket one persistent kernel: each block sorts whole segments in
+//      LDS (the blocksort of mergesort.hxx: odd-even network + in-LDS merge
+//      passes) and takes the next segment of its bucket;
+//   4. large segments: a segmented LSD radix sort over 12288-key tiles that
+//      never straddle segments.  Counts are laid out [segment][digit][tile],
+//      so one exclusive scan gives every key's position in the concatenation
+//      of the large segments; position - (start of its segment in that
+//      concatenation) + (segment start) is its place in the array.
+//      Descending order uses inverted bits; float keys map -0.0 to +0.0 so
+//      ties follow comparator semantics.
+#pragma once
+#include "mergesort.hxx"
+#include "radix.hxx"
+#include "scan.hxx"
+
+namespace mgpu {
+namespace hip {
+
+// Blocksort shapes available to the buckets, by capacity.
+struct segsort_shape_t { int nt, vt; };
+constexpr int segsort_num_shapes = 8;
+constexpr segsort_shape_t segsort_shapes[segsort_num_shapes] = {
+    {64, 4}, {64, 8}, {128, 8}, {64, 16}, {256, 8}, {256, 16}, {512, 8}, {512, 16}};
+constexpr int segsort_max_buckets = 6;
+// Large-segment radix tiles.
+constexpr int segsort_radix_nt = 512, segsort_radix_vt = 24, segsort_radix_cnt = 1024;
+constexpr int segsort_radix_nv = segsort_radix_nt * segsort_radix_vt;
+
+// A bucket table: buckets hold segments of length <= cap[b] (ascending); each
+// bucket is sorted by blocksort shape `shape[b]` (index into segsort_shapes,
+// capacity >= cap[b]).  Longer segments go to the large bucket.
+struct segsort_tuning_t {
+  int num_buckets;
+  int cap[segsort_max_buckets];
+  int shape[segsort_max_buckets];
+};
+
+inline segsort_tuning_t segsort_default_tuning() {
+  segsort_tuning_t t;
+  t.num_buckets = 6;
+  const int caps[6] = {256, 512, 1024, 2048, 4096, 8192};
+  const int shapes[6] = {0, 1, 2, 4, 5, 7};
+  for (int b = 0; b < 6; ++b) {
+    t.cap[b] = caps[b];
+    t.shape[b] = shapes[b];
+  }
+  return t;
+}
+
+struct segsort_bucket_caps_t { int cap[segsort_max_buckets]; int num_buckets; };
+
+__device__ inline void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
+                                      int64_t i, int64_t& begin, int64_t& end) {
+  begin = segments[i];
+  end = i + 1 < num_segments ? (int64_t)segments[i + 1] : count;
+}
+
+// 1. flags[b * S + i] for b in [0, num_buckets] (num_buckets = large), and
+//    large_tiles[i] = tiles of segment i if it is large, else 0.
+template <int NT>
+__global__ __launch_bounds__(NT) void kernel_segsort_classify(
+    const int32_t* segments, int64_t num_segments, int64_t count, segsort_bucket_caps_t caps,
+    uint32_t* flags, uint32_t* large_tiles) {
+  const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (i >= num_segments) return;
+  int64_t b0, b1;
+  segment_bounds(segments, num_segments, count, i, b0, b1);
+  const int64_t len = b1 - b0;
+  int bucket = -1;
+  if (len > 1) {
+    bucket = caps.num_buckets;
+    for (int b = caps.num_buckets - 1; b >= 0; --b)
+      if (len <= caps.cap[b]) bucket = b;
+  }
+  for (int b = 0; b <= caps.num_buckets; ++b)
+    flags[b * num_segments + i] = bucket == b ? 1u : 0u;
+  large_tiles[i] = bucket == caps.num_buckets ? (uint32_t)div_up(len, (int64_t)segsort_radix_nv) : 0u;
+}
+
+// 2. list[pos[k]] = segment of flag k; bounds[b] = {start, end} of bucket b's
+//    list; large_tasks[tile_pos[i] + t] = (i, t) for every tile t of a large
+//    segment i; *num_tasks = total large tiles.
+template <int NT>
+__global__ __launch_bounds__(NT) void kernel_segsort_compact(
+    const uint32_t* flags, const uint32_t* pos, int64_t num_segments, int num_buckets,
+    int32_t* list, int32_t* bounds, const uint32_t* large_tiles, const uint32_t* tile_pos,
+    int2* large_tasks, int32_t* num_tasks) {
+  const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+  const int64_t total = (int64_t)(num_buckets + 1) * num_segments;
+  if (k < total && flags[k]) list[pos[k]] = (int32_t)(k % num_segments);
+  if (k <= num_buckets) {
+    const int b = (int)k;
+    bounds[2 * b] = (int32_t)pos[(int64_t)b * num_segments];
+    bounds[2 * b + 1] = b == num_buckets ? (int32_t)(pos[total - 1] + flags[total - 1])
+                                         : (int32_t)pos[(int64_t)(b + 1) * num_segments];
+  }
+  if (k < num_segments) {
+    const uint32_t n = large_tiles[k];
+    for (uint32_t t = 0; t < n; ++t) large_tasks[tile_pos[k] + t] = make_int2((int)k, (int)t);
+    if (k == num_segments - 1) *num_tasks = (int32_t)(tile_pos[k] + n);
+  }
+}
+
+// 3. Sort whole segments of one bucket in LDS, block b taking list entries
+//    bounds[0] + b, + grid, ...
+template <int NT, int VT, typename key_t, typename val_t, typename comp_t>
+__global__ __launch_bounds__(NT) void kernel_segsort_blocks(
+    key_t* keys, val_t* vals, int64_t count, const int32_t* segments, int64_t num_segments,
+    const int32_t* list, const int32_t* bounds, comp_t comp) {
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  __shared__ blocksort_storage_t<NT, VT, key_t, val_t> s;
+  const int32_t first = bounds[0], last = bounds[1];
+  for (int64_t e = first + (int64_t)blockIdx.x; e < last; e += gridDim.x) {
+    int64_t b0, b1;
+    segment_bounds(segments, num_segments, count, list[e], b0, b1);
+    val_t* v = has_values ? vals + b0 : vals;
+    cta_blocksort<NT, VT, false>(keys + b0, v, keys + b0, v, (int)(b1 - b0), 0, comp, s);
+    __syncthreads();
+  }
+}
+
+// Sort bits for the large-segment radix: radix order with -0.0 tied to +0.0
+// for floats (comparator order), inverted for descending comparators.
+template <typename key_t, bool descending>
+struct segsort_bits_t {
+  typedef typename radix_permute_t<key_t>::unsigned_type type;
+  __device__ type operator()(key_t k) const {
+    if constexpr (std::is_floating_point_v<key_t>) {
+      if (k == key_t(0)) k = key_t(0);
+    }
+    type b = radix_permute_t<key_t>::to_radix_bits(k);
+    return descending ? (type)~b : b;
+  }
+};
+
+// 4a. Counts of one digit for every large-segment tile task:
+//     counts[256 * tile_pos[seg] + d * tiles(seg) + t].
+template <typename key_t, bool descending>
+__global__ __launch_bounds__(segsort_radix_cnt) void kernel_segsort_radix_count(
+    const key_t* keys, int64_t count, const int32_t* segments, int64_t num_segments,
+    const int2* tasks, const int32_t* num_tasks, const uint32_t* large_tiles,
+    const uint32_t* tile_pos, int shift, uint32_t* counts) {
+  __shared__ uint32_t h[radix_num_bins * 16];
+  const int32_t n = *num_tasks;
+  for (int64_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const int2 task = tasks[e];
+    int64_t b0, b1;
+    segment_bounds(segments, num_segments, count, task.x, b0, b1);
+    const int64_t base = b0 + (int64_t)task.y * segsort_radix_nv;
+    const int tile_count = (int)min_<int64_t>(segsort_radix_nv, b1 - base);
+    const uint32_t tiles = large_tiles[task.x];
+    cta_radix_count<segsort_radix_cnt, segsort_radix_nv>(
+        keys + base, tile_count, shift, segsort_bits_t<key_t, descending>(),
+        counts + 256ull * tile_pos[task.x] + task.y, tiles, h);
+    __syncthreads();
+  }
+}
+
+// 4b. Scatter of one digit for every large-segment tile task.
+template <typename key_t, bool has_values, bool descending>
+__global__ __launch_bounds__(segsort_radix_nt) void kernel_segsort_radix_scatter(
+    const key_t* keys_in, key_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+    int64_t count, const int32_t* segments, int64_t num_segments, const int2* tasks,
+    const int32_t* num_tasks, const uint32_t* large_tiles, const uint32_t* tile_pos, int shift,
+    const uint32_t* offsets) {
+  __shared__ radix_scatter_storage_t<segsort_radix_nt, segsort_radix_vt, key_t> s;
+  const int tid = threadIdx.x;
+  const int32_t n = *num_tasks;
+  for (int64_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const int2 task = tasks[e];
+    int64_t b0, b1;
+    segment_bounds(segments, num_segments, count, task.x, b0, b1);
+    const int64_t base = b0 + (int64_t)task.y * segsort_radix_nv;
+    const int tile_count = (int)min_<int64_t>(segsort_radix_nv, b1 - base);
+    const uint32_t tiles = large_tiles[task.x];
+    const uint64_t cbase = 256ull * tile_pos[task.x];
+    // Position of the segment's first key in the concatenation of the large
+    // segments (digit 0, tile 0 of this segment), mapped back to b0.
+    const uint32_t seg_first = offsets[cbase];
+    uint32_t goff = 0;
+    if (tid < radix_num_bins)
+      goff = offsets[cbase + (uint64_t)tid * tiles + task.y] - seg_first + (uint32_t)b0;
+    cta_radix_scatter<segsort_radix_nt, segsort_radix_vt, key_t, has_values>(
+        keys_in + base, has_values ? vals_in + base : vals_in, keys_out, vals_out, tile_count,
+        shift, segsort_bits_t<key_t, descending>(), goff, count, s);
+    __syncthreads();
+  }
+}
+
+// Persistent grids: enough blocks to fill the chip, never more than the work.
+constexpr int segsort_grid_blocks = 2048;
+
+// Upper bound of large-segment tiles: at most count / (largest capacity + 1)
+// large segments, each adding at most one partial tile.
+inline int64_t segsort_max_tiles(int64_t count, const segsort_tuning_t& t) {
+  return div_up(count, (int64_t)segsort_radix_nv) + count / (t.cap[t.num_buckets - 1] + 1) + 1;
+}
+
+template <typename key_t, typename val_t>
+inline size_t segmented_sort_aux_bytes(int64_t count, int64_t num_segments,
+                                       const segsort_tuning_t& tuning) {
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  const int64_t S = num_segments, B = tuning.num_buckets + 1;
+  const int64_t max_tiles = segsort_max_tiles(count, tuning);
+  const int64_t max_counts = 256 * max_tiles;
+  size_t need = 0;
+  need += aux_bytes<uint32_t>(B * S) * 2;         // flags, pos
+  need += aux_bytes<int32_t>(B * S);              // lists
+  need += aux_bytes<int32_t>(2 * B);              // bounds
+  need += aux_bytes<uint32_t>(S) * 2;             // large tiles, tile_pos
+  need += aux_bytes<int2>(max_tiles);             // tasks
+  need += aux_bytes<int32_t>(1);                  // number of tasks
+  need += aux_bytes<uint32_t>(max_counts);        // counts / offsets
+  need += aux_bytes<key_t>(count);                // radix ping-pong keys
+  if (has_values) need += aux_bytes<uint32_t>(count);
+  size_t scan_need = 0;
+  scan<256, 16>(nullptr, scan_need, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                max(max(B * S, S), max_counts), MGPU_SCAN_EXCLUSIVE, 0u);
+  return need + scan_need;
+}
+
+template <int SHAPE, typename key_t, typename val_t, typename comp_t>
+inline hipError_t segsort_launch_bucket(key_t* keys, val_t* vals, int64_t count,
+                                        const int32_t* segments, int64_t num_segments,
+                                        const int32_t* list, const int32_t* bounds, comp_t comp,
+                                        hipStream_t stream) {
+  constexpr int NT = segsort_shapes[SHAPE].nt, VT = segsort_shapes[SHAPE].vt;
+  const unsigned grid = (unsigned)min_<int64_t>(num_segments, segsort_grid_blocks);
+  return launch_kernel("mgpu_segsort_blocks", kernel_segsort_blocks<NT, VT, key_t, val_t, comp_t>,
+                       dim3(grid), dim3(NT), 0, stream, keys, vals, count, segments,
+                       num_segments, list, bounds, comp);
+}
+
+template <typename key_t, typename val_t, typename comp_t>
+inline hipError_t segsort_dispatch_shape(int shape, key_t* keys, val_t* vals, int64_t count,
+                                         const int32_t* segments, int64_t num_segments,
+                                         const int32_t* list, const int32_t* bounds,
+                                         comp_t comp, hipStream_t stream) {
+#define MGPU_SEGSORT_SHAPE(i)                                                             \
+  case i:                                                                                 \
+    return segsort_launch_bucket<i>(keys, vals, count, segments, num_segments, list,     \
+                                    bounds, comp, stream);
+  switch (shape) {
+    MGPU_SEGSORT_SHAPE(0) MGPU_SEGSORT_SHAPE(1) MGPU_SEGSORT_SHAPE(2) MGPU_SEGSORT_SHAPE(3)
+    MGPU_SEGSORT_SHAPE(4) MGPU_SEGSORT_SHAPE(5) MGPU_SEGSORT_SHAPE(6) MGPU_SEGSORT_SHAPE(7)
+  }
+#undef MGPU_SEGSORT_SHAPE
+  return hipErrorInvalidValue;
+}
+
+// Stable per-segment sort; val_t = uint32_t or empty_t.  descending selects
+// the large-segment bit order matching comp (less: false, greater: true).
+template <bool descending, typename key_t, typename val_t, typename comp_t>
+mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
+                             val_t* vals, int64_t count, const int32_t* segments,
+                             int64_t num_segments, comp_t comp, const segsort_tuning_t& tuning) {
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  static_assert(!has_values || std::is_same_v<val_t, uint32_t>, "values are uint32");
+  if (count < 0 || num_segments < 0) return MGPU_INVALID_ARG;
+  if (tuning.num_buckets < 1 || tuning.num_buckets > segsort_max_buckets) return MGPU_INVALID_ARG;
+  for (int b = 0; b < tuning.num_buckets; ++b) {
+    if (tuning.shape[b] < 0 || tuning.shape[b] >= segsort_num_shapes) return MGPU_INVALID_ARG;
+    const segsort_shape_t sh = segsort_shapes[tuning.shape[b]];
+    if (tuning.cap[b] < 2 || tuning.cap[b] > sh.nt * sh.vt) return MGPU_INVALID_ARG;
+    if (b && tuning.cap[b] <= tuning.cap[b - 1]) return MGPU_INVALID_ARG;
+  }
+  if (count == 0 || num_segments == 0) return MGPU_OK;
+  if (count >= ((int64_t)1 << 31) || num_segments >= ((int64_t)1 << 31) / (segsort_max_buckets + 1))
+    return MGPU_UNSUPPORTED;
+  const size_t need = segmented_sort_aux_bytes<key_t, val_t>(count, num_segments, tuning);
+  if (!aux_data) {
+    aux_size += need;
+    return MGPU_OK;
+  }
+  if (!keys || !segments || (has_values && !vals)) return MGPU_INVALID_ARG;
+  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+
+  const int NB = tuning.num_buckets;
+  const int64_t S = num_segments, B = NB + 1;
+  const int64_t max_tiles = segsort_max_tiles(count, tuning);
+  const int64_t max_counts = 256 * max_tiles;
+  void* p = aux_data;
+  uint32_t* flags = advance_pointer<uint32_t>(p, B * S);
+  uint32_t* pos = advance_pointer<uint32_t>(p, B * S);
+  int32_t* list = advance_pointer<int32_t>(p, B * S);
+  int32_t* bounds = advance_pointer<int32_t>(p, 2 * B);
+  uint32_t* large_tiles = advance_pointer<uint32_t>(p, S);
+  uint32_t* tile_pos = advance_pointer<uint32_t>(p, S);
+  int2* tasks = advance_pointer<int2>(p, max_tiles);
+  int32_t* num_tasks = advance_pointer<int32_t>(p, 1);
+  uint32_t* counts = advance_pointer<uint32_t>(p, max_counts);
+  key_t* keys2 = advance_pointer<key_t>(p, count);
+  uint32_t* vals2 = has_values ? advance_pointer<uint32_t>(p, count) : nullptr;
+  size_t scan_cap = aux_size - ((char*)p - (char*)aux_data);
+
+  segsort_bucket_caps_t caps;
+  caps.num_buckets = NB;
+  for (int b = 0; b < segsort_max_buckets; ++b) caps.cap[b] = b < NB ? tuning.cap[b] : 0;
+
+  hipError_t e = launch_kernel("mgpu_segsort_classify", kernel_segsort_classify<256>,
+                               dim3((unsigned)div_up(S, (int64_t)256)), dim3(256), 0, stream,
+                               segments, S, count, caps, flags, large_tiles);
+  if (e != hipSuccess) return hip_status(e);
+  mgpu_status_t st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)flags, pos, B * S,
+                                   MGPU_SCAN_EXCLUSIVE, 0u);
+  if (st != MGPU_OK) return st;
+  st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)large_tiles, tile_pos, S,
+                     MGPU_SCAN_EXCLUSIVE, 0u);
+  if (st != MGPU_OK) return st;
+  e = launch_kernel("mgpu_segsort_compact", kernel_segsort_compact<256>,
+                    dim3((unsigned)div_up(B * S, (int64_t)256)), dim3(256), 0, stream,
+                    (const uint32_t*)flags, (const uint32_t*)pos, S, NB, list, bounds,
+                    (const uint32_t*)large_tiles, (const uint32_t*)tile_pos, tasks, num_tasks);
+  if (e != hipSuccess) return hip_status(e);
+
+  // Blocksort buckets.
+  for (int b = 0; b < NB; ++b) {
+    e = segsort_dispatch_shape(tuning.shape[b], keys, vals, count, segments, S, list,
+                               bounds + 2 * b, comp, stream);
+    if (e != hipSuccess) return hip_status(e);
+  }
+
+  // Large segments: segmented LSD radix, ping-pong keys <-> keys2.
+  constexpr int D = (int)sizeof(key_t);
+  key_t* k_in = keys;
+  key_t* k_out = keys2;
+  uint32_t* v_in = has_values ? (uint32_t*)vals : nullptr;
+  uint32_t* v_out = vals2;
+  const unsigned grid = (unsigned)min_<int64_t>(max_tiles, segsort_grid_blocks);
+  for (int pass = 0; pass < D; ++pass) {
+    // Every count a task owns is rewritten each pass; entries past the last
+    // task only follow the real ones in the scan, so they need no clearing.
+    const int shift = 8 * pass;
+    e = launch_kernel("mgpu_segsort_radix_count", kernel_segsort_radix_count<key_t, descending>,
+                      dim3(grid), dim3(segsort_radix_cnt), 0, stream, (const key_t*)k_in, count,
+                      segments, S, (const int2*)tasks, (const int32_t*)num_tasks,
+                      (const uint32_t*)large_tiles, (const uint32_t*)tile_pos, shift, counts);
+    if (e != hipSuccess) return hip_status(e);
+    st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)counts, counts, max_counts,
+                       MGPU_SCAN_EXCLUSIVE, 0u);
+    if (st != MGPU_OK) return st;
+    e = launch_kernel("mgpu_segsort_radix_scatter",
+                      kernel_segsort_radix_scatter<key_t, has_values, descending>, dim3(grid),
+                      dim3(segsort_radix_nt), 0, stream, (const key_t*)k_in, k_out,
+                      (const uint32_t*)v_in, v_out, count, segments, S, (const int2*)tasks,
+                      (const int32_t*)num_tasks, (const uint32_t*)large_tiles,
+                      (const uint32_t*)tile_pos, shift, (const uint32_t*)counts);
+    if (e != hipSuccess) return hip_status(e);
+    std::swap(k_in, k_out);
+    std::swap(v_in, v_out);
+  }
+  return MGPU_OK;
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Autotuner: for every size class (cap[b-1], cap[b]] of the default bucket
+// table, time each blocksort shape that can hold cap[b] on synthetic
+// segments with lengths uniform over the class (SplitMix64 keys), and keep
+// the fastest.  Returns the table (also usable as an explicit argument).
+
+template <typename key_t>
+__global__ __launch_bounds__(256) void kernel_segsort_tune_fill(key_t* keys, uint32_t* vals,
+                                                               int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)i + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    if constexpr (std::is_floating_point_v<key_t>) keys[i] = (key_t)(z >> 40) * (key_t)(1.0 / 16777216.0);
+    else keys[i] = (key_t)z;
+    if (vals) vals[i] = (uint32_t)i;
+  }
+}
+
+template <typename key_t, typename val_t, typename comp_t>
+inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_tuning_t& out,
+                                      double* ms_table /* [buckets][shapes] or null */) {
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  out = segsort_default_tuning();
+  const int64_t target = (int64_t)1 << 22;  // keys per trial
+  key_t* keys = nullptr;
+  uint32_t* vals = nullptr;
+  int32_t* segs = nullptr;
+  int32_t* list = nullptr;
+  int32_t* bounds = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  mgpu_status_t status = MGPU_OK;
+  auto check = [&](hipError_t e) {
+    if (e != hipSuccess && status == MGPU_OK) status = hip_status(e);
+    return e == hipSuccess;
+  };
+  const int64_t max_segs = target / 2 + 1;
+  if (!check(hipMalloc(&keys, target * sizeof(key_t))) ||
+      !check(hipMalloc(&vals, target * sizeof(uint32_t))) ||
+      !check(hipMalloc(&segs, max_segs * sizeof(int32_t))) ||
+      !check(hipMalloc(&list, max_segs * sizeof(int32_t))) ||
+      !check(hipMalloc(&bounds, 2 * sizeof(int32_t))) || !check(hipEventCreate(&e0)) ||
+      !check(hipEventCreate(&e1))) {
+    (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(segs); (void)hipFree(list);
+    (void)hipFree(bounds);
+    return status;
+  }
+  std::vector<int32_t> h_segs, h_list;
+  uint64_t rng = 0x243F6A8885A308D3ull;
+  auto next_rand = [&]() {
+    rng += 0x9E3779B97F4A7C15ull;
+    uint64_t z = rng;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  for (int b = 0; b < out.num_buckets && status == MGPU_OK; ++b) {
+    const int lo = b ? out.cap[b - 1] + 1 : 2, hi = out.cap[b];
+    h_segs.clear();
+    int64_t n = 0;
+    while (n < target - hi) {
+      h_segs.push_back((int32_t)n);
+      n += lo + (int64_t)(next_rand() % (uint64_t)(hi - lo + 1));
+    }
+    const int64_t S = (int64_t)h_segs.size();
+    h_list.resize(S);
+    for (int64_t i = 0; i < S; ++i) h_list[i] = (int32_t)i;
+    const int32_t h_bounds[2] = {0, (int32_t)S};
+    check(hipMemcpy(segs, h_segs.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
+    check(hipMemcpy(list, h_list.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
+    check(hipMemcpy(bounds, h_bounds, sizeof(h_bounds), hipMemcpyHostToDevice));
+    double best = 1e30;
+    for (int sh = 0; sh < segsort_num_shapes && status == MGPU_OK; ++sh) {
+      const int capacity = segsort_shapes[sh].nt * segsort_shapes[sh].vt;
+      double ms = -1;
+      if (capacity >= hi && capacity <= 8 * hi) {
+        float total = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+          hipLaunchKernelGGL(kernel_segsort_tune_fill<key_t>, dim3(1024), dim3(256), 0, stream, keys,
+                             has_values ? vals : nullptr, n, (uint64_t)rep);
+          check(hipEventRecord(e0, stream));
+          check(segsort_dispatch_shape(sh, keys, has_values ? (val_t*)vals : (val_t*)nullptr, n,
+                                       segs, S, list, bounds, comp, stream));
+          check(hipEventRecord(e1, stream));
+          check(hipEventSynchronize(e1));
+          float t = 0;
+          check(hipEventElapsedTime(&t, e0, e1));
+          if (rep) total += t;  // the first trial warms up
+        }
+        ms = total / 2;
+        if (ms < best) {
+          best = ms;
+          out.shape[b] = sh;
+        }
+      }
+      if (ms_table) ms_table[b * segsort_num_shapes + sh] = ms;
+    }
+  }
+  (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(segs); (void)hipFree(list);
+  (void)hipFree(bounds); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  return status;
+}
+
+}  // namespace hip
+}  // namespace mgpu

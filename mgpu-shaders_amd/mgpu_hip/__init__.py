@@ -88,6 +88,29 @@ def load():
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, vp, i64, c_int, ct]
         f = getattr(lib, "mgpu_reduce_%s" % t)
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, vp, ct]
+    class SegsortTuning(ctypes.Structure):
+        _fields_ = [("num_buckets", ctypes.c_int), ("cap", ctypes.c_int * 6),
+                    ("shape", ctypes.c_int * 6)]
+    global _SegsortTuning
+    _SegsortTuning = SegsortTuning
+    tp = ctypes.POINTER(SegsortTuning)
+    for k in KEY_TYPES:
+        f = getattr(lib, "mgpu_segmented_sort_keys_%s" % k)
+        f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, vp, i64, c_int, tp]
+        f = getattr(lib, "mgpu_segmented_sort_kv_%s_u32" % k)
+        f.restype, f.argtypes = c_int, [vp, sz, vp, vp, vp, i64, vp, i64, c_int, tp]
+        f = getattr(lib, "mgpu_segmented_sort_autotune_%s" % k)
+        f.restype, f.argtypes = c_int, [vp, c_int, c_int, tp, ctypes.POINTER(ctypes.c_double)]
+    lib.mgpu_segsort_shapes.restype = c_int
+    lib.mgpu_segsort_shapes.argtypes = [ctypes.POINTER(c_int), ctypes.POINTER(c_int), c_int]
+    lib.mgpu_capture_begin.restype = c_int
+    lib.mgpu_capture_begin.argtypes = [vp]
+    lib.mgpu_capture_end.restype = c_int
+    lib.mgpu_capture_end.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.mgpu_graph_launch.restype = c_int
+    lib.mgpu_graph_launch.argtypes = [vp, vp]
+    lib.mgpu_graph_destroy.restype = None
+    lib.mgpu_graph_destroy.argtypes = [vp]
     lib.mgpu_generate.restype = c_int
     lib.mgpu_generate.argtypes = [vp, vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
     lib.mgpu_set_lookback_spin_limit.restype = None
@@ -280,6 +303,101 @@ def reduce(input, out, init=0, elem_type=None, aux=None, stream=None):
     fn = getattr(load(), "mgpu_reduce_%s" % t)
     return _run_two_phase(fn, [_ptr(input), input.numel(), _ptr(out), _CTYPE[t](init)],
                           "reduce", aux, stream)
+
+
+# ---------------------------------------------------------------------------
+# Segmented sort (batched; segments[i] = first element of segment i)
+
+_SegsortTuning = None
+
+
+def segsort_shapes():
+    """[(nt, vt)] of the blocksort shapes a bucket can use."""
+    lib = load()
+    nt = (ctypes.c_int * 16)()
+    vt = (ctypes.c_int * 16)()
+    n = lib.mgpu_segsort_shapes(nt, vt, 16)
+    return [(nt[i], vt[i]) for i in range(n)]
+
+
+def _tuning_struct(tuning):
+    if tuning is None:
+        return None
+    t = _SegsortTuning()
+    t.num_buckets = len(tuning["cap"])
+    for b, (c, sh) in enumerate(zip(tuning["cap"], tuning["shape"])):
+        t.cap[b] = c
+        t.shape[b] = sh
+    return ctypes.byref(t)
+
+
+def segmented_sort(keys, segments, vals=None, comp="less", key_type=None, tuning=None, aux=None,
+                   stream=None):
+    """Stable sort of every segment of keys (+ vals) in place; segments is an
+    int32 device tensor of segment starts."""
+    _device_check(keys, vals, segments)
+    if segments.dtype != torch.int32:
+        raise ValueError("segments must be int32")
+    k = _key_type(keys, key_type)
+    lib = load()
+    t = _tuning_struct(tuning)
+    if vals is None:
+        fn = getattr(lib, "mgpu_segmented_sort_keys_%s" % k)
+        args = [_ptr(keys), keys.numel(), _ptr(segments), segments.numel(), _comp(comp), t]
+    else:
+        if vals.numel() != keys.numel() or vals.element_size() != 4:
+            raise ValueError("vals must be 32-bit with one per key")
+        fn = getattr(lib, "mgpu_segmented_sort_kv_%s_u32" % k)
+        args = [_ptr(keys), _ptr(vals), keys.numel(), _ptr(segments), segments.numel(),
+                _comp(comp), t]
+    return _run_two_phase(fn, args, "segmented_sort", aux, stream)
+
+
+def segmented_sort_autotune(key_type="u32", has_values=True, comp="less", stream=None):
+    """Time every blocksort shape per size bucket on this device; the chosen
+    table becomes the default for this key/value/comparator and is returned
+    as {"cap": [...], "shape": [...], "ms": [[...]]}."""
+    lib = load()
+    t = _SegsortTuning()
+    nshapes = len(segsort_shapes())
+    ms = (ctypes.c_double * (6 * nshapes))()
+    _check(getattr(lib, "mgpu_segmented_sort_autotune_%s" % key_type)(
+        _stream(stream), 1 if has_values else 0, _comp(comp), ctypes.byref(t), ms),
+        "segmented_sort_autotune")
+    nb = t.num_buckets
+    return {"cap": [t.cap[b] for b in range(nb)], "shape": [t.shape[b] for b in range(nb)],
+            "ms": [[ms[b * nshapes + i] for i in range(nshapes)] for b in range(nb)]}
+
+
+# ---------------------------------------------------------------------------
+# Graph capture of library calls
+
+class Graph:
+    """Record library calls made on `stream` inside `with Graph(stream) as g:`
+    and replay them with g.launch()."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.handle = ctypes.c_void_p(0)
+
+    def __enter__(self):
+        _check(load().mgpu_capture_begin(_stream(self.stream)), "capture_begin")
+        return self
+
+    def __exit__(self, *exc):
+        st = load().mgpu_capture_end(_stream(self.stream), ctypes.byref(self.handle))
+        if exc[0] is None:
+            _check(st, "capture_end")
+        return False
+
+    def launch(self, stream=None):
+        _check(load().mgpu_graph_launch(self.handle, _stream(stream or self.stream)),
+               "graph_launch")
+
+    def __del__(self):
+        if self.handle and _lib is not None:
+            _lib.mgpu_graph_destroy(self.handle)
+            self.handle = ctypes.c_void_p(0)
 
 
 # ---------------------------------------------------------------------------

@@ -9,6 +9,7 @@ Parity status of each function (pinned by tests/test_oracle.py):
   generate            pinned: SplitMix64 known-answer values + numpy restatement
   glibc_rand          pinned: glibc TYPE_3 additive-feedback known answer
   stable_sort         pinned: numpy stable argsort (independent implementation)
+  segmented_stable_sort pinned: numpy lexsort by (segment, key)
   stable_sort_radix   pinned: numpy stable argsort of the bit keys; float
                       special values ordered by IEEE total order
   merge               pinned: numpy stable argsort of the concatenation
@@ -50,6 +51,7 @@ def lib():
         L.oracle_glibc_rand.argtypes = [i64, vp]
         L.oracle_stable_sort.argtypes = [c_int, vp, vp, i64, c_int]
         L.oracle_stable_sort_radix.argtypes = [c_int, vp, vp, i64]
+        L.oracle_segmented_stable_sort.argtypes = [c_int, vp, vp, i64, vp, i64, c_int]
         L.oracle_std_sort_u32.argtypes = [vp, i64]
         L.oracle_merge.argtypes = [c_int, vp, vp, i64, vp, vp, i64, vp, vp, c_int]
         for n in ("oracle_exclusive_scan_u32", "oracle_inclusive_scan_u32"):
@@ -116,6 +118,15 @@ def stable_sort(keys, vals=None, kt="u32", greater=False):
     k = np.ascontiguousarray(keys, dtype=NP[kt]).copy()
     v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
     lib().oracle_stable_sort(KT[kt], _p(k), _p(v), len(k), int(greater))
+    return (k, v) if vals is not None else k
+
+
+def segmented_stable_sort(keys, segments, vals=None, kt="u32", greater=False):
+    """std::stable_sort of every segment (segments = int32 starts)."""
+    k = np.ascontiguousarray(keys, dtype=NP[kt]).copy()
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    sg = np.ascontiguousarray(segments, dtype=np.int32)
+    lib().oracle_segmented_stable_sort(KT[kt], _p(k), _p(v), len(k), _p(sg), len(sg), int(greater))
     return (k, v) if vals is not None else k
 
 

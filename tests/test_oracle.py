@@ -349,3 +349,25 @@ def test_vk_radix2_known_answer_input():
     keys = oracle.glibc_rand(n)
     s = oracle.std_sort_u32(keys)
     np.testing.assert_array_equal(s, np.sort(keys))
+
+
+@pytest.mark.parametrize("kt", KTS)
+@pytest.mark.parametrize("greater", [False, True])
+def test_segmented_stable_sort_vs_numpy_lexsort(kt, greater):
+    rng = np.random.default_rng(5)
+    lens = rng.integers(0, 300, 200)
+    lens[::17] = 0
+    lens[::13] = 1
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int32)
+    n = int(lens.sum())
+    keys = keys_for(kt, n, 7)[np.arange(n) % 50]
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = oracle.segmented_stable_sort(keys, starts, vals, kt, greater)
+    seg_id = np.repeat(np.arange(len(lens)), lens)
+    if greater:
+        key_rank = ~keys if kt in ("u32", "u64") else -keys.astype(np.float64)
+    else:
+        key_rank = keys
+    order = np.lexsort((key_rank, seg_id))
+    np.testing.assert_array_equal(v, vals[order])
+    np.testing.assert_array_equal(k, keys[order])
