@@ -272,6 +272,48 @@ def bench_segsort(torch, mgpu_hip, steps, warmup):
     return out
 
 
+def bench_serving(torch, mgpu_hip, count=256):
+    """Serving engine under an open-loop load: sort / merge / scan requests
+    with key types u32 / u64 / f32 and total sizes log-uniform over
+    [2^10, 2^26], 8 slots, 64 GiB budget.  Saturation = all requests
+    submitted at once; then Poisson arrivals at 0.5x, 0.8x and 1.1x of the
+    saturation request rate.  Latency = completion - scheduled arrival.
+    Every response is verified after its run (outside the timed window)."""
+    from mgpu_hip import serving
+
+    def one(rate, seed):
+        reqs = serving.make_requests(count, seed=seed)
+        sv = serving.Server(slots=8, budget_bytes=64 << 30)
+        try:
+            resps = serving.run_load(sv, reqs, rate=rate)
+            summ = serving.summarize(resps, reqs)
+            worst = 0.0
+            for r in resps:
+                worst = max(worst, serving.verify(r))
+                sv.release(r["id"])
+            summ["verified"] = len(resps)
+            summ["f32_scan_max_rel_err"] = worst
+            summ["peak_budget_gib"] = sv.stats()["peak"] / 2 ** 30
+        finally:
+            sv.close()
+        torch.cuda.synchronize()
+        return summ
+
+    one(None, 99)  # warm-up: pool growth, code objects
+    sat = one(None, 1)
+    log("serving saturation", sat)
+    out = {"requests_per_run": count, "slots": 8, "budget_gib": 64,
+           "sizes": "total elements log-uniform [2^10, 2^26]", "saturation": sat, "rates": []}
+    for i, f in enumerate((0.5, 0.8, 1.1)):
+        rate = f * sat["requests_per_s"]
+        summ = one(rate, 2 + i)
+        summ["load_factor"] = f
+        summ["offered_requests_per_s"] = rate
+        log("serving", f, {k: summ[k] for k in ("p50_ms", "p99_ms", "keys_per_s")})
+        out["rates"].append(summ)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -354,6 +396,7 @@ def main():
         torch.cuda.empty_cache()
         result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
         torch.cuda.empty_cache()
+        result["serving"] = bench_serving(torch, mgpu_hip)
     if not args.no_cpu:
         log("cpu baseline 2^%d" % args.cpu_log2)
         result["cpu_baseline"] = cpu_baseline(args.cpu_log2)

@@ -136,6 +136,9 @@ MGPU_DECL_RADIX(f32, float)
 MGPU_DECL_SCAN(i32, int32_t)
 MGPU_DECL_SCAN(u32, uint32_t)
 MGPU_DECL_SCAN(f32, float)
+/* u64 (modulo 2^64): reduce-then-scan in three phases (vk/scan.hxx:65-113),
+ * since a 64-bit sum and a look-back flag do not fit one atomic word.      */
+MGPU_DECL_SCAN(u64, uint64_t)
 
 /* ------------------------------------------------------------------------ */
 /* Batched segmented sort (no counterpart in the reference, which has no
@@ -188,6 +191,63 @@ mgpu_status_t mgpu_graph_launch(mgpu_graph_t graph, mgpu_stream_t stream);
 void mgpu_graph_destroy(mgpu_graph_t graph);
 
 /* ------------------------------------------------------------------------ */
+/* Serving engine (no counterpart in the reference): continuous batching of
+ * sort / merge / scan requests over `slots` streams.  A request is admitted
+ * as soon as a slot is free and its device bytes fit the HBM budget (first
+ * fit from the head of the queue); its input is generated on the device from
+ * (kind per key type, seed), and its result (and, with keep_inputs, its
+ * input) stays device-resident until mgpu_server_release.  Requests:
+ *   SORT   stable key-value LSD radix sort of n keys, values = index;
+ *          keys uniform (f32: finite, U[-1, 1))
+ *   MERGE  stable key-value merge of two ascending arrays of n keys each
+ *          (values 0..2n-1 in A then B order)
+ *   SCAN   u32/i32/u64: exclusive (mod 2^bits) of uniform keys;
+ *          f32: inclusive of U[0, 1) (multiples of 2^-24)
+ * Timestamps are host steady-clock milliseconds since creation.            */
+typedef struct mgpu_server_st* mgpu_server_t;
+typedef enum mgpu_req_op_t { MGPU_REQ_SORT = 0, MGPU_REQ_MERGE = 1, MGPU_REQ_SCAN = 2 } mgpu_req_op_t;
+typedef enum mgpu_key_type_t {
+  MGPU_KEY_U32 = 0, MGPU_KEY_I32 = 1, MGPU_KEY_U64 = 2, MGPU_KEY_F32 = 3
+} mgpu_key_type_t;
+typedef struct mgpu_request_t {
+  int32_t op;        /* mgpu_req_op_t   */
+  int32_t key_type;  /* mgpu_key_type_t */
+  int64_t n;         /* elements (merge: of each input) */
+  uint64_t seed;
+} mgpu_request_t;
+typedef struct mgpu_response_t {
+  int64_t id;
+  int32_t status;    /* mgpu_status_t of the enqueue */
+  int32_t op, key_type, slot;
+  int64_t n;
+  void* keys;        /* device: sorted keys, merged keys or scan output */
+  void* vals;        /* device: u32 values (sort, merge) or NULL          */
+  void* input_a;     /* device: input keys (merge: A), NULL unless kept   */
+  void* input_b;     /* device: merge B keys, NULL unless kept            */
+  double t_submit_ms, t_start_ms, t_done_ms;
+} mgpu_response_t;
+mgpu_status_t mgpu_server_create(int slots, size_t hbm_budget, int keep_inputs,
+                                 mgpu_server_t* out);
+double mgpu_server_now(mgpu_server_t server);
+/* Device bytes the server budgets for a request.                          */
+size_t mgpu_server_request_bytes(const mgpu_request_t* req);
+mgpu_status_t mgpu_server_submit(mgpu_server_t server, const mgpu_request_t* req, int64_t* id);
+/* Completed responses not yet reported (non-blocking); returns how many.  */
+int mgpu_server_poll(mgpu_server_t server, mgpu_response_t* out, int max);
+/* Blocks until every submitted request has completed (MGPU_OK), or until
+ * nothing is in flight and no pending request fits the budget that
+ * unreleased results hold (MGPU_AUX_TOO_SMALL: release, then drain again).
+ * mgpu_server_submit returns MGPU_UNSUPPORTED for a request larger than the
+ * whole budget or with n >= 2^30.                                          */
+mgpu_status_t mgpu_server_drain(mgpu_server_t server);
+/* Frees a reported response's device buffers and returns its bytes to the
+ * budget.  The caller's own reads of those buffers must have completed.    */
+mgpu_status_t mgpu_server_release(mgpu_server_t server, int64_t id);
+void mgpu_server_stats(mgpu_server_t server, size_t* in_use, size_t* peak, int64_t* pending,
+                       int64_t* in_flight);
+void mgpu_server_destroy(mgpu_server_t server);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic inputs (no counterpart in the reference, whose examples fill
  * host buffers with rand(), e.g. examples/vk_radix2/vk_radix2.cxx:25-27).
  * Counter-based SplitMix64: element i of `kind` for `seed`, written for
@@ -200,7 +260,10 @@ typedef enum mgpu_gen_kind_t {
   MGPU_GEN_F32_01 = 3,     /* float32 multiples of 2^-24 in [0, 1)           */
   MGPU_GEN_U64 = 4,        /* uniform over [0, 2^64)                         */
   MGPU_GEN_F32_PM1 = 5,    /* float32 multiples of 2^-23 in [-1, 1)          */
-  MGPU_GEN_IOTA = 6        /* element i = i                                  */
+  MGPU_GEN_IOTA = 6,       /* element i = i                                  */
+  MGPU_GEN_SORTED_U32 = 7, /* ascending: i*s + rand % s, s = 2^32 / count    */
+  MGPU_GEN_SORTED_U64 = 8, /* ascending: i*s + rand % s, s = 2^64 / count    */
+  MGPU_GEN_SORTED_F32 = 9  /* ascending: (i + U[0,1)) / count                */
 } mgpu_gen_kind_t;
 mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
                             mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset);

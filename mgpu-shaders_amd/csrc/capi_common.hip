@@ -67,8 +67,8 @@ int mgpu_timing_collect(const char** names, int* launches, double* total_ms, int
 mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
                             mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset) {
   if (count < 0 || (elem_bytes != 4 && elem_bytes != 8)) return MGPU_INVALID_ARG;
-  if ((int)kind < 0 || (int)kind > (int)gen_iota) return MGPU_INVALID_ARG;
-  if (kind == MGPU_GEN_U64 && elem_bytes != 8) return MGPU_INVALID_ARG;
+  if ((int)kind < 0 || (int)kind > (int)gen_sorted_f32) return MGPU_INVALID_ARG;
+  if ((kind == MGPU_GEN_U64 || kind == MGPU_GEN_SORTED_U64) && elem_bytes != 8) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
   if (!out) return MGPU_INVALID_ARG;
   int64_t blocks = min_<int64_t>(div_up(count, (int64_t)256), 8192);

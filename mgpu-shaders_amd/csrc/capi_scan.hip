@@ -29,3 +29,25 @@ using namespace mgpu::hip;
 MGPU_DEF_SCAN(i32, int32_t)
 MGPU_DEF_SCAN(u32, uint32_t)
 MGPU_DEF_SCAN(f32, float)
+
+extern "C" mgpu_status_t mgpu_scan_u64(void* aux, size_t* aux_size, mgpu_stream_t stream,
+                                       const uint64_t* input, uint64_t* output, int64_t count,
+                                       mgpu_scan_type_t type, uint64_t init) {
+  if (!aux_size) return MGPU_INVALID_ARG;
+  return scan64<uint64_t>(aux, *aux_size, (hipStream_t)stream, input, output, count, type, init);
+}
+
+extern "C" mgpu_status_t mgpu_scan_exclusive_u64(void* aux, size_t* aux_size,
+                                                 mgpu_stream_t stream, uint64_t* data,
+                                                 int64_t count, uint64_t init) {
+  if (!aux_size) return MGPU_INVALID_ARG;
+  return scan64<uint64_t>(aux, *aux_size, (hipStream_t)stream, (const uint64_t*)data, data, count,
+                          MGPU_SCAN_EXCLUSIVE, init);
+}
+
+extern "C" mgpu_status_t mgpu_reduce_u64(void* aux, size_t* aux_size, mgpu_stream_t stream,
+                                         const uint64_t* input, int64_t count, uint64_t* out,
+                                         uint64_t init) {
+  if (!aux_size) return MGPU_INVALID_ARG;
+  return reduce<256>(aux, *aux_size, (hipStream_t)stream, input, count, out, init);
+}

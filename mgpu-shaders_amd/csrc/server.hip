@@ -1,0 +1,447 @@
+// Serving engine: continuous batching of sort / merge / scan requests.
+//
+// No counterpart in the reference (it is a library of primitives with no
+// request loop).  One worker thread owns `slots` HIP streams.  A request is
+// admitted as soon as a slot is free and its device bytes fit the HBM
+// budget (first fit from the head of the queue, so small requests do not
+// wait behind a large one that does not fit yet); its buffers come from a
+// stream-ordered memory pool, its input is generated on the device, and its
+// result stays device-resident until the client releases it.  Requests on
+// different slots run concurrently on the GPU.  Completion is detected with
+// HIP events; timestamps are host steady-clock milliseconds since the
+// server was created.
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "mgpu/hip/generate.hxx"
+#include "mgpu/hip/merge.hxx"
+#include "mgpu/hip/radix.hxx"
+#include "mgpu/hip/scan.hxx"
+#include "tuning.hxx"
+
+using namespace mgpu::hip;
+
+namespace {
+
+typedef std::chrono::steady_clock clock_t_;
+
+struct rec_t {
+  int64_t id = 0;
+  mgpu_request_t req{};
+  double t_submit = 0, t_start = 0, t_done = 0;
+  int slot = -1;
+  mgpu_status_t status = MGPU_OK;
+  size_t bytes = 0;
+  void* in_a = nullptr;   // keys (sort, scan input) or A keys (merge)
+  void* in_b = nullptr;   // B keys (merge)
+  void* out = nullptr;    // result keys / scan output
+  void* vals = nullptr;   // sort / merge values (u32)
+  void* vals_a = nullptr; // merge input values
+  void* aux = nullptr;
+  hipEvent_t done = nullptr;
+  bool reported = false;
+};
+
+size_t key_bytes(int kt) { return kt == MGPU_KEY_U64 ? 8 : 4; }
+
+}  // namespace
+
+struct mgpu_server_st {
+  std::mutex m;
+  std::condition_variable cv;
+  std::deque<rec_t*> pending;
+  std::vector<rec_t*> slot_rec;
+  std::vector<hipStream_t> streams;
+  hipStream_t free_stream = nullptr;
+  std::map<int64_t, rec_t*> recs;
+  std::vector<rec_t*> completed;
+  size_t budget = 0, in_use = 0, peak = 0;
+  int64_t next_id = 0;
+  int64_t in_flight = 0;
+  bool stop = false;
+  int keep_inputs = 1;
+  int device = 0;
+  clock_t_::time_point t0;
+  std::thread worker;
+  hipMemPool_t pool = nullptr;
+
+  double now() const {
+    return std::chrono::duration<double, std::milli>(clock_t_::now() - t0).count();
+  }
+};
+
+namespace {
+
+// Device bytes a request needs (buffers + aux), from the sizing calls.
+size_t request_bytes(const mgpu_request_t& r) {
+  const int64_t n = r.n;
+  const size_t kb = key_bytes(r.key_type);
+  size_t aux = 0;
+  switch (r.op) {
+    case MGPU_REQ_SORT:
+      if (r.key_type == MGPU_KEY_U64)
+        radix_sort<512, 12, 512>(nullptr, aux, nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
+      else
+        radix_sort<512, 24, 1024>(nullptr, aux, nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
+      return 2 * aux_round(n * kb) + aux_round(n * 4) + aux_round(aux);
+    case MGPU_REQ_MERGE:
+      merge<256, 15>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n,
+                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, (uint32_t*)nullptr,
+                     (uint32_t*)nullptr, less_t<uint32_t>());
+      return 2 * aux_round(n * kb) + aux_round(2 * n * 4) + aux_round(2 * n * kb) +
+             aux_round(2 * n * 4) + aux_round(aux);
+    case MGPU_REQ_SCAN:
+      if (r.key_type == MGPU_KEY_U64)
+        scan64<uint64_t>(nullptr, aux, nullptr, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
+      else
+        scan<256, 16>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
+                      MGPU_SCAN_EXCLUSIVE, 0u);
+      return 2 * aux_round(n * kb) + aux_round(aux);
+  }
+  return 0;
+}
+
+hipError_t gen(hipStream_t s, void* out, int64_t n, int eb, int kind, uint64_t seed) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = min_<int64_t>(div_up(n, (int64_t)256), 8192);
+  return launch_kernel("mgpu_generate", kernel_generate<256>, dim3((unsigned)blocks), dim3(256), 0,
+                       s, out, n, eb, kind, seed, (uint64_t)0);
+}
+
+hipError_t alloc(rec_t* r, void** p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) bytes = 256;
+  return hipMallocAsync(p, aux_round(bytes), s);
+}
+
+// Allocate, generate and enqueue one request on stream s.
+mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
+  const mgpu_request_t& q = r->req;
+  const int64_t n = q.n;
+  const int kb = (int)key_bytes(q.key_type);
+  hipError_t e = hipSuccess;
+  auto ok = [&](hipError_t x) {
+    if (x != hipSuccess && e == hipSuccess) e = x;
+    return x == hipSuccess;
+  };
+  size_t aux = 0;
+  switch (q.op) {
+    case MGPU_REQ_SORT: {
+      const int kind = q.key_type == MGPU_KEY_U64 ? gen_u64 : q.key_type == MGPU_KEY_F32 ? gen_f32_pm1 : gen_u32;
+      if (q.key_type == MGPU_KEY_U64)
+        radix_sort<512, 12, 512>(nullptr, aux, s, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
+      else
+        radix_sort<512, 24, 1024>(nullptr, aux, s, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
+      if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->out, n * kb, s)) ||
+          !ok(alloc(r, &r->vals, n * 4, s)) || !ok(alloc(r, &r->aux, aux, s)))
+        break;
+      ok(gen(s, r->in_a, n, kb, kind, q.seed));
+      ok(hipMemcpyAsync(r->out, r->in_a, n * kb, hipMemcpyDeviceToDevice, s));
+      ok(gen(s, r->vals, n, 4, gen_iota, 0));
+      if (e != hipSuccess) break;
+      size_t cap = aux_round(aux ? aux : 1);
+      mgpu_status_t st;
+      switch (q.key_type) {
+        case MGPU_KEY_U64:
+          st = radix_sort<512, 12, 512>(r->aux, cap, s, (uint64_t*)r->out, (uint32_t*)r->vals, n);
+          break;
+        case MGPU_KEY_I32:
+          st = radix_sort<512, 24, 1024>(r->aux, cap, s, (int32_t*)r->out, (uint32_t*)r->vals, n);
+          break;
+        case MGPU_KEY_F32:
+          st = radix_sort<512, 24, 1024>(r->aux, cap, s, (float*)r->out, (uint32_t*)r->vals, n);
+          break;
+        default:
+          st = radix_sort<512, 24, 1024>(r->aux, cap, s, (uint32_t*)r->out, (uint32_t*)r->vals, n);
+      }
+      return st;
+    }
+    case MGPU_REQ_MERGE: {
+      const int kind = q.key_type == MGPU_KEY_U64 ? gen_sorted_u64
+                       : q.key_type == MGPU_KEY_F32 ? gen_sorted_f32 : gen_sorted_u32;
+      merge<256, 15>(nullptr, aux, s, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n,
+                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, (uint32_t*)nullptr,
+                     (uint32_t*)nullptr, less_t<uint32_t>());
+      if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->in_b, n * kb, s)) ||
+          !ok(alloc(r, &r->vals_a, 2 * n * 4, s)) || !ok(alloc(r, &r->out, 2 * n * kb, s)) ||
+          !ok(alloc(r, &r->vals, 2 * n * 4, s)) || !ok(alloc(r, &r->aux, aux, s)))
+        break;
+      ok(gen(s, r->in_a, n, kb, kind, q.seed));
+      ok(gen(s, r->in_b, n, kb, kind, q.seed + 1));
+      ok(gen(s, r->vals_a, 2 * n, 4, gen_iota, 0));
+      if (e != hipSuccess) break;
+      size_t cap = aux_round(aux ? aux : 1);
+      const uint32_t* va = (const uint32_t*)r->vals_a;
+      const uint32_t* vb = va + n;
+      uint32_t* vc = (uint32_t*)r->vals;
+      switch (q.key_type) {
+        case MGPU_KEY_U64:
+          return merge<256, 15>(r->aux, cap, s, (const uint64_t*)r->in_a, va, n,
+                                (const uint64_t*)r->in_b, vb, n, (uint64_t*)r->out, vc,
+                                less_t<uint64_t>());
+        case MGPU_KEY_F32:
+          return merge<256, 15>(r->aux, cap, s, (const float*)r->in_a, va, n,
+                                (const float*)r->in_b, vb, n, (float*)r->out, vc, less_t<float>());
+        case MGPU_KEY_I32:
+          return merge<256, 15>(r->aux, cap, s, (const int32_t*)r->in_a, va, n,
+                                (const int32_t*)r->in_b, vb, n, (int32_t*)r->out, vc,
+                                less_t<int32_t>());
+        default:
+          return merge<256, 15>(r->aux, cap, s, (const uint32_t*)r->in_a, va, n,
+                                (const uint32_t*)r->in_b, vb, n, (uint32_t*)r->out, vc,
+                                less_t<uint32_t>());
+      }
+    }
+    case MGPU_REQ_SCAN: {
+      const int kind = q.key_type == MGPU_KEY_U64 ? gen_u64 : q.key_type == MGPU_KEY_F32 ? gen_f32_01 : gen_u32;
+      if (q.key_type == MGPU_KEY_U64)
+        scan64<uint64_t>(nullptr, aux, s, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
+      else
+        scan<256, 16>(nullptr, aux, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
+                      MGPU_SCAN_EXCLUSIVE, 0u);
+      if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->out, n * kb, s)) ||
+          !ok(alloc(r, &r->aux, aux, s)))
+        break;
+      ok(gen(s, r->in_a, n, kb, kind, q.seed));
+      if (e != hipSuccess) break;
+      size_t cap = aux_round(aux ? aux : 1);
+      switch (q.key_type) {
+        case MGPU_KEY_U64:
+          return scan64<uint64_t>(r->aux, cap, s, (const uint64_t*)r->in_a, (uint64_t*)r->out, n,
+                                  MGPU_SCAN_EXCLUSIVE, 0ull);
+        case MGPU_KEY_F32:
+          return scan<256, 16>(r->aux, cap, s, (const float*)r->in_a, (float*)r->out, n,
+                               MGPU_SCAN_INCLUSIVE, 0.0f);
+        default:
+          return scan<256, 16>(r->aux, cap, s, (const uint32_t*)r->in_a, (uint32_t*)r->out, n,
+                               MGPU_SCAN_EXCLUSIVE, 0u);
+      }
+    }
+  }
+  return e == hipSuccess ? MGPU_INVALID_ARG : hip_status(e);
+}
+
+void free_buffers(mgpu_server_st* sv, rec_t* r) {
+  for (void** p : {&r->in_a, &r->in_b, &r->out, &r->vals, &r->vals_a, &r->aux})
+    if (*p) {
+      (void)hipFreeAsync(*p, sv->free_stream);
+      *p = nullptr;
+    }
+}
+
+void worker_loop(mgpu_server_st* sv) {
+  (void)hipSetDevice(sv->device);
+  std::unique_lock<std::mutex> lock(sv->m);
+  while (true) {
+    bool progressed = false;
+    // Completions.
+    for (size_t i = 0; i < sv->slot_rec.size(); ++i) {
+      rec_t* r = sv->slot_rec[i];
+      if (!r) continue;
+      hipError_t q = hipEventQuery(r->done);
+      if (q == hipErrorNotReady) continue;
+      r->t_done = sv->now();
+      if (q != hipSuccess && r->status == MGPU_OK) r->status = hip_status(q);
+      if (!sv->keep_inputs) {
+        for (void** p : {&r->in_a, &r->in_b, &r->vals_a})
+          if (*p) { (void)hipFreeAsync(*p, sv->free_stream); *p = nullptr; }
+      }
+      if (r->aux) { (void)hipFreeAsync(r->aux, sv->free_stream); r->aux = nullptr; }
+      sv->completed.push_back(r);
+      sv->slot_rec[i] = nullptr;
+      --sv->in_flight;
+      progressed = true;
+      sv->cv.notify_all();
+    }
+    // Admissions: first fit from the head of the queue into free slots.
+    for (size_t i = 0; i < sv->slot_rec.size(); ++i) {
+      if (sv->slot_rec[i]) continue;
+      auto it = sv->pending.begin();
+      for (; it != sv->pending.end(); ++it)
+        if (sv->in_use + (*it)->bytes <= sv->budget) break;
+      if (it == sv->pending.end()) break;
+      rec_t* r = *it;
+      sv->pending.erase(it);
+      sv->in_use += r->bytes;
+      sv->peak = std::max(sv->peak, sv->in_use);
+      r->slot = (int)i;
+      r->t_start = sv->now();
+      hipStream_t s = sv->streams[i];
+      r->status = enqueue(r, s);
+      (void)hipEventRecord(r->done, s);
+      sv->slot_rec[i] = r;
+      ++sv->in_flight;
+      progressed = true;
+    }
+    if (sv->stop && sv->pending.empty() && sv->in_flight == 0) break;
+    if (!progressed) {
+      if (sv->in_flight == 0)
+        sv->cv.wait_for(lock, std::chrono::milliseconds(5));
+      else {
+        lock.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+        lock.lock();
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+mgpu_status_t mgpu_server_create(int slots, size_t hbm_budget, int keep_inputs,
+                                 mgpu_server_t* out) {
+  if (!out || slots < 1 || slots > 64 || hbm_budget == 0) return MGPU_INVALID_ARG;
+  *out = nullptr;
+  mgpu_server_st* sv = new mgpu_server_st();
+  sv->budget = hbm_budget;
+  sv->keep_inputs = keep_inputs;
+  sv->t0 = clock_t_::now();
+  hipError_t e = hipGetDevice(&sv->device);
+  (void)lookback_config();
+  if (e == hipSuccess) e = hipDeviceGetDefaultMemPool(&sv->pool, sv->device);
+  if (e == hipSuccess) {
+    uint64_t threshold = ~0ull;  // keep freed memory in the pool
+    e = hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  }
+  for (int i = 0; e == hipSuccess && i < slots; ++i) {
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) sv->streams.push_back(s);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->free_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    for (hipStream_t s : sv->streams) (void)hipStreamDestroy(s);
+    delete sv;
+    return hip_status(e);
+  }
+  sv->slot_rec.assign(slots, nullptr);
+  sv->worker = std::thread(worker_loop, sv);
+  *out = sv;
+  return MGPU_OK;
+}
+
+double mgpu_server_now(mgpu_server_t sv) { return sv ? sv->now() : 0.0; }
+
+size_t mgpu_server_request_bytes(const mgpu_request_t* req) {
+  return req ? request_bytes(*req) : 0;
+}
+
+mgpu_status_t mgpu_server_submit(mgpu_server_t sv, const mgpu_request_t* req, int64_t* id) {
+  if (!sv || !req || req->n < 1) return MGPU_INVALID_ARG;
+  if (req->op < MGPU_REQ_SORT || req->op > MGPU_REQ_SCAN) return MGPU_INVALID_ARG;
+  if (req->key_type < MGPU_KEY_U32 || req->key_type > MGPU_KEY_F32) return MGPU_INVALID_ARG;
+  if (req->n >= ((int64_t)1 << 30)) return MGPU_UNSUPPORTED;
+  rec_t* r = new rec_t();
+  r->req = *req;
+  r->bytes = request_bytes(*req);
+  if (r->bytes > sv->budget) {
+    delete r;
+    return MGPU_UNSUPPORTED;
+  }
+  if (hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess) {
+    delete r;
+    return MGPU_HIP_ERROR;
+  }
+  std::lock_guard<std::mutex> lock(sv->m);
+  r->id = sv->next_id++;
+  r->t_submit = sv->now();
+  sv->recs[r->id] = r;
+  sv->pending.push_back(r);
+  if (id) *id = r->id;
+  sv->cv.notify_all();
+  return MGPU_OK;
+}
+
+int mgpu_server_poll(mgpu_server_t sv, mgpu_response_t* out, int max) {
+  if (!sv) return 0;
+  std::lock_guard<std::mutex> lock(sv->m);
+  int n = 0;
+  while (n < max && !sv->completed.empty()) {
+    rec_t* r = sv->completed.front();
+    sv->completed.erase(sv->completed.begin());
+    mgpu_response_t& o = out[n++];
+    o.id = r->id;
+    o.status = r->status;
+    o.op = r->req.op;
+    o.key_type = r->req.key_type;
+    o.n = r->req.n;
+    o.keys = r->out;
+    o.vals = r->vals;
+    o.input_a = r->in_a;
+    o.input_b = r->in_b;
+    o.t_submit_ms = r->t_submit;
+    o.t_start_ms = r->t_start;
+    o.t_done_ms = r->t_done;
+    o.slot = r->slot;
+    r->reported = true;
+  }
+  return n;
+}
+
+mgpu_status_t mgpu_server_drain(mgpu_server_t sv) {
+  if (!sv) return MGPU_INVALID_ARG;
+  std::unique_lock<std::mutex> lock(sv->m);
+  // Nothing in flight and nothing pending that fits: only a release can
+  // make progress, so waiting longer would never end.
+  auto stuck = [&] {
+    for (rec_t* r : sv->pending)
+      if (sv->in_use + r->bytes <= sv->budget) return false;
+    return true;
+  };
+  sv->cv.wait(lock, [&] { return sv->in_flight == 0 && (sv->pending.empty() || stuck()); });
+  return sv->pending.empty() ? MGPU_OK : MGPU_AUX_TOO_SMALL;
+}
+
+mgpu_status_t mgpu_server_release(mgpu_server_t sv, int64_t id) {
+  if (!sv) return MGPU_INVALID_ARG;
+  std::lock_guard<std::mutex> lock(sv->m);
+  auto it = sv->recs.find(id);
+  if (it == sv->recs.end() || !it->second->reported) return MGPU_INVALID_ARG;
+  rec_t* r = it->second;
+  free_buffers(sv, r);
+  (void)hipEventDestroy(r->done);
+  sv->in_use -= r->bytes;
+  sv->recs.erase(it);
+  delete r;
+  sv->cv.notify_all();
+  return MGPU_OK;
+}
+
+void mgpu_server_stats(mgpu_server_t sv, size_t* in_use, size_t* peak, int64_t* pending,
+                       int64_t* in_flight) {
+  if (!sv) return;
+  std::lock_guard<std::mutex> lock(sv->m);
+  if (in_use) *in_use = sv->in_use;
+  if (peak) *peak = sv->peak;
+  if (pending) *pending = (int64_t)sv->pending.size();
+  if (in_flight) *in_flight = sv->in_flight;
+}
+
+void mgpu_server_destroy(mgpu_server_t sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lock(sv->m);
+    sv->stop = true;
+    sv->cv.notify_all();
+  }
+  sv->worker.join();
+  for (auto& kv : sv->recs) {
+    free_buffers(sv, kv.second);
+    (void)hipEventDestroy(kv.second->done);
+    delete kv.second;
+  }
+  (void)hipStreamSynchronize(sv->free_stream);
+  (void)hipMemPoolTrimTo(sv->pool, 0);  // hand the pooled memory back
+  for (hipStream_t s : sv->streams) (void)hipStreamDestroy(s);
+  (void)hipStreamDestroy(sv->free_stream);
+  delete sv;
+}
+
+}  // extern "C"

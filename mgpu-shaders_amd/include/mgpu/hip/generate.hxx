@@ -10,6 +10,10 @@
 //   u64         u32(0, i) << 32 | u32(1, i)
 //   f32_pm1     (u32(0, i) >> 8) * 2^-23 - 1      (exact, in [-1, 1))
 //   iota        i
+//   sorted_u32  i * s + (u32(0, i) % s), s = max(1, 2^32 / n)   (ascending)
+//   sorted_u64  i * s + (u64 % s),       s = max(1, 2^64 / n)   (ascending)
+//   sorted_f32  (i + (u32(0, i) >> 8) * 2^-24) / n               (ascending)
+// (sorted kinds use n = the generated count and ignore the offset)
 #pragma once
 #include "meta.hxx"
 
@@ -24,6 +28,9 @@ enum gen_kind_t : int {
   gen_u64 = 4,
   gen_f32_pm1 = 5,
   gen_iota = 6,
+  gen_sorted_u32 = 7,
+  gen_sorted_u64 = 8,
+  gen_sorted_f32 = 9,
 };
 
 __host__ __device__ inline uint64_t splitmix64_mix(uint64_t z) {
@@ -36,9 +43,24 @@ __host__ __device__ inline uint32_t gen_word32(uint64_t seed, uint64_t stream, u
   return (uint32_t)(splitmix64_mix(c) >> 32);
 }
 
-// Element i of a generator kind, as raw bits (4 or 8 bytes).
-__host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_t i) {
+// Element i of a generator kind, as raw bits (4 or 8 bytes); n is the
+// length of the generated array (used by the sorted kinds).
+__host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_t i, uint64_t n = 1) {
   switch (kind) {
+    case gen_sorted_u32: {
+      uint64_t s = n ? (((uint64_t)1 << 32) / n) : 1;
+      s = s ? s : 1;
+      return i * s + gen_word32(seed, 0, i) % s;
+    }
+    case gen_sorted_u64: {
+      uint64_t s = n > 1 ? (~0ull / n) : ~0ull;
+      uint64_t r = ((uint64_t)gen_word32(seed, 0, i) << 32) | gen_word32(seed, 1, i);
+      return i * s + r % s;
+    }
+    case gen_sorted_f32: {
+      float f = ((float)i + (float)(gen_word32(seed, 0, i) >> 8) * (1.0f / 16777216.0f)) / (float)n;
+      uint32_t b; std::memcpy(&b, &f, 4); return b;
+    }
     case gen_u32: return gen_word32(seed, 0, i);
     case gen_lowentropy:
       return gen_word32(seed, 0, i) & gen_word32(seed, 1, i) & gen_word32(seed, 2, i) &
@@ -63,7 +85,8 @@ __global__ __launch_bounds__(NT) void kernel_generate(void* out, int64_t count, 
                                                       int kind, uint64_t seed, uint64_t offset) {
   int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < count; i += stride) {
-    uint64_t v = gen_element(kind, seed, offset + (uint64_t)i);
+    uint64_t v = kind >= gen_sorted_u32 ? gen_element(kind, seed, (uint64_t)i, (uint64_t)count)
+                                        : gen_element(kind, seed, offset + (uint64_t)i);
     if (elem_bytes == 8) ((uint64_t*)out)[i] = v;
     else ((uint32_t*)out)[i] = (uint32_t)v;
   }

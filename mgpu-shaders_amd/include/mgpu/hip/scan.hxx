@@ -316,6 +316,7 @@ template <typename T> struct reduce_traits;
 template <> struct reduce_traits<int32_t> { typedef uint32_t acc_t; };
 template <> struct reduce_traits<uint32_t> { typedef uint32_t acc_t; };
 template <> struct reduce_traits<float> { typedef double acc_t; };
+template <> struct reduce_traits<uint64_t> { typedef uint64_t acc_t; };
 
 template <int NT, typename acc_t>
 __device__ inline acc_t block_reduce(acc_t v, acc_t* s_warp) {
@@ -336,34 +337,32 @@ __global__ __launch_bounds__(NT) void kernel_reduce_partial(
     typename reduce_traits<T>::acc_t* partials) {
   typedef typename reduce_traits<T>::acc_t acc_t;
   typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+  constexpr int EPV = 16 / (int)sizeof(T);  // elements per 128-bit vector
   __shared__ acc_t s_warp[NT / warp_size];
   const int tid = threadIdx.x;
   acc_t sum = 0;
-  const int64_t nvec = (count - head) / 4;
+  const int64_t nvec = (count - head) / EPV;
   const u32x4* vp = (const u32x4*)(input + head);
   const int64_t stride = (int64_t)gridDim.x * NT;
   int64_t i = (int64_t)blockIdx.x * NT + tid;
-  auto add4 = [&](u32x4 v) {
-    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  auto addv = [&](u32x4 v) {
+    T e[EPV];
+    std::memcpy(e, &v, 16);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      T t;
-      std::memcpy(&t, &w[e], 4);
-      sum += (acc_t)t;
-    }
+    for (int q = 0; q < EPV; ++q) sum += (acc_t)e[q];
   };
   for (; i + 3 * stride < nvec; i += 4 * stride) {
     u32x4 v0 = __builtin_nontemporal_load(vp + i);
     u32x4 v1 = __builtin_nontemporal_load(vp + i + stride);
     u32x4 v2 = __builtin_nontemporal_load(vp + i + 2 * stride);
     u32x4 v3 = __builtin_nontemporal_load(vp + i + 3 * stride);
-    add4(v0); add4(v1); add4(v2); add4(v3);
+    addv(v0); addv(v1); addv(v2); addv(v3);
   }
-  for (; i < nvec; i += stride) add4(__builtin_nontemporal_load(vp + i));
+  for (; i < nvec; i += stride) addv(__builtin_nontemporal_load(vp + i));
   if (blockIdx.x == 0) {
     if (tid < head) sum += (acc_t)input[tid];
-    int64_t t = head + nvec * 4 + tid;
-    if (tid < 4 && t < count) sum += (acc_t)input[t];
+    int64_t t = head + nvec * EPV + tid;
+    if (tid < EPV && t < count) sum += (acc_t)input[t];
   }
   acc_t total = block_reduce<NT>(sum, s_warp);
   if (tid == 0) partials[blockIdx.x] = total;
@@ -409,6 +408,141 @@ mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const
   e = launch_kernel("mgpu_reduce_final", kernel_reduce_final<NT, T>, dim3(1), dim3(NT), 0,
                     stream, (const acc_t*)partials, (int)blocks, out, init);
   return hip_status(e);
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// 64-bit integer scan (modulo 2^64).  A 64-bit sum and a status flag do not
+// fit one atomic word, so this scan is three phases in the reference's own
+// structure (vk/scan.hxx:65-113): tile reductions, an exclusive scan of the
+// tile partials (recursively, by the same three phases), then a downsweep
+// that rescans each tile with its carry.  12 bytes moved per element.
+
+template <int NT, int VT, typename T>
+__global__ __launch_bounds__(NT) void kernel_scan64_reduce(const T* input, int64_t count,
+                                                           T* partials) {
+  constexpr int NV = NT * VT;
+  __shared__ T s_warp[NT / warp_size];
+  const int64_t base = (int64_t)blockIdx.x * NV;
+  T sum = 0;
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    int64_t idx = base + i * NT + threadIdx.x;
+    if (idx < count) sum += input[idx];
+  }
+  T total = block_reduce<NT>(sum, s_warp);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Exclusive block scan of NT*VT elements per tile with a carry per tile;
+// carries == nullptr means carry 0 (single tile).  Thread-serial over VT
+// consecutive elements, then a wave scan, then the wave totals.
+template <int NT, int VT, bool inclusive, typename T>
+__global__ __launch_bounds__(NT) void kernel_scan64_down(const T* input, T* output, int64_t count,
+                                                         const T* carries, T init) {
+  constexpr int NV = NT * VT;
+  constexpr int NW = NT / warp_size;
+  __shared__ T s_tile[NV];
+  __shared__ T s_wave[NW];
+  const int tid = threadIdx.x, lane = lane_id(), warp = tid / warp_size;
+  const int64_t base = (int64_t)blockIdx.x * NV;
+  // Coalesced load, transpose through LDS to VT consecutive per thread.
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    int64_t idx = base + i * NT + tid;
+    s_tile[i * NT + tid] = idx < count ? input[idx] : T(0);
+  }
+  __syncthreads();
+  T x[VT];
+  T run = 0;
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    x[i] = s_tile[tid * VT + i];
+    run += x[i];
+  }
+  // Inclusive wave scan of the per-thread totals (64-bit, via shuffles).
+  T incl = run;
+#pragma unroll
+  for (int off = 1; off < warp_size; off <<= 1) {
+    T y = __shfl_up(incl, off, warp_size);
+    if (lane >= off) incl += y;
+  }
+  if (lane == warp_size - 1) s_wave[warp] = incl;
+  __syncthreads();
+  T carry = (carries ? carries[blockIdx.x] : T(0)) + init;
+  for (int w = 0; w < warp; ++w) carry += s_wave[w];
+  carry += incl - run;
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    T v = x[i];
+    x[i] = inclusive ? carry + v : carry;
+    carry += v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < VT; ++i) s_tile[tid * VT + i] = x[i];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < VT; ++i) {
+    int64_t idx = base + i * NT + tid;
+    if (idx < count) output[idx] = s_tile[i * NT + tid];
+  }
+}
+
+template <typename T>
+inline size_t scan64_aux_bytes(int64_t count) {
+  constexpr int NV = 256 * 8;
+  size_t need = 0;
+  for (int64_t n = count; n > NV; n = div_up(n, (int64_t)NV)) need += aux_bytes<T>(div_up(n, (int64_t)NV));
+  return need;
+}
+
+template <typename T>
+inline hipError_t scan64_run(void* p, const T* input, T* output, int64_t count, bool inclusive,
+                             T init, hipStream_t stream) {
+  constexpr int NT = 256, VT = 8, NV = NT * VT;
+  const int64_t tiles = div_up(count, (int64_t)NV);
+  hipError_t e;
+  if (tiles == 1) {
+    if (inclusive)
+      return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, true, T>, dim3(1),
+                           dim3(NT), 0, stream, input, output, count, (const T*)nullptr, init);
+    return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, false, T>, dim3(1),
+                         dim3(NT), 0, stream, input, output, count, (const T*)nullptr, init);
+  }
+  T* partials = (T*)p;
+  void* rest = (char*)p + aux_bytes<T>(tiles);
+  e = launch_kernel("mgpu_scan64_reduce", kernel_scan64_reduce<NT, VT, T>, dim3((unsigned)tiles),
+                    dim3(NT), 0, stream, input, count, partials);
+  if (e != hipSuccess) return e;
+  e = scan64_run<T>(rest, (const T*)partials, partials, tiles, false, T(0), stream);
+  if (e != hipSuccess) return e;
+  if (inclusive)
+    return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, true, T>,
+                         dim3((unsigned)tiles), dim3(NT), 0, stream, input, output, count,
+                         (const T*)partials, init);
+  return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, false, T>,
+                       dim3((unsigned)tiles), dim3(NT), 0, stream, input, output, count,
+                       (const T*)partials, init);
+}
+
+// Scan of 64-bit integers (uint64_t / int64_t, wrapping modulo 2^64).
+template <typename T>
+mgpu_status_t scan64(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
+                     T* output, int64_t count, mgpu_scan_type_t type, T init) {
+  static_assert(sizeof(T) == 8, "64-bit element types");
+  if (count < 0) return MGPU_INVALID_ARG;
+  if (type != MGPU_SCAN_EXCLUSIVE && type != MGPU_SCAN_INCLUSIVE) return MGPU_INVALID_ARG;
+  if (count == 0) return MGPU_OK;
+  if (div_up(count, (int64_t)2048) >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
+  const size_t need = scan64_aux_bytes<T>(count);
+  if (!aux_data) {
+    aux_size += need;
+    return MGPU_OK;
+  }
+  if (!input || !output) return MGPU_INVALID_ARG;
+  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  return hip_status(scan64_run<T>(aux_data, input, output, count, type == MGPU_SCAN_INCLUSIVE,
+                                  init, stream));
 }
 
 }  // namespace hip

@@ -26,7 +26,7 @@ MGPU_OK, MGPU_INVALID_ARG, MGPU_AUX_TOO_SMALL, MGPU_HIP_ERROR, MGPU_UNSUPPORTED 
 LESS, GREATER = 0, 1
 EXCLUSIVE, INCLUSIVE = 0, 1
 GEN_KINDS = {"u32": 0, "lowentropy": 1, "i32_0_16": 2, "f32_01": 3, "u64": 4, "f32_pm1": 5,
-             "iota": 6}
+             "iota": 6, "sorted_u32": 7, "sorted_u64": 8, "sorted_f32": 9}
 KEY_TYPES = ("u32", "i32", "u64", "f32")
 _CTYPE = {"u32": ctypes.c_uint32, "i32": ctypes.c_int32, "u64": ctypes.c_uint64,
           "f32": ctypes.c_float}
@@ -80,7 +80,7 @@ def load():
             f = getattr(lib, name % k)
             f.restype = c_int
             f.argtypes = args
-    for t in ("i32", "u32", "f32"):
+    for t in ("i32", "u32", "f32", "u64"):
         ct = _CTYPE[t]
         f = getattr(lib, "mgpu_scan_exclusive_%s" % t)
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, ct]
@@ -265,7 +265,7 @@ def radix_sort(keys, vals=None, key_type=None, aux=None, stream=None):
 # ---------------------------------------------------------------------------
 # Scan / reduce (inc/mgpu/vk/scan.hxx:10-115)
 
-_SCAN_T = {torch.int32: "i32", torch.uint32: "u32", torch.float32: "f32"}
+_SCAN_T = {torch.int32: "i32", torch.uint32: "u32", torch.float32: "f32", torch.uint64: "u64"}
 
 
 def _scan_type(t, elem_type):

@@ -157,3 +157,21 @@ def test_scan_forced_lookback_fallback(limit, inplace):
     np.testing.assert_array_equal(to_host(out).view(np.uint32), oracle.exclusive_scan_u32(x))
     exact = oracle.inclusive_scan_f64(f)
     assert np.max(np.abs(to_host(outf).astype(np.float64) - exact) / exact) <= F32_SCAN_RTOL
+
+
+@pytest.mark.parametrize("n", [1, 7, 2047, 2048, 2049, 2048 * 2048 + 3, (1 << 22) + 1])
+@pytest.mark.parametrize("inclusive", [False, True])
+def test_scan_u64_three_phase(n, inclusive):
+    banner("scan u64", n, inclusive)
+    x = oracle.generate(n, "u64", 30)
+    d = to_dev(x)
+    out = torch.empty_like(d)
+    mgpu_hip.scan_into(d, out, inclusive=inclusive, init=12345)
+    with np.errstate(over="ignore"):
+        inc = np.cumsum(x, dtype=np.uint64) + np.uint64(12345)
+    exp = inc if inclusive else np.concatenate([np.array([12345], np.uint64), inc[:-1]])
+    np.testing.assert_array_equal(to_host(out), exp)
+    r = torch.zeros(1, dtype=torch.uint64, device="cuda")
+    mgpu_hip.reduce(d, r, init=5)
+    with np.errstate(over="ignore"):
+        assert int(to_host(r)[0]) == int(np.sum(x, dtype=np.uint64) + np.uint64(5))
