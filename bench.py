@@ -177,7 +177,13 @@ def secondary(torch, mgpu_hip, steps, warmup):
          "mgpu_reduce_partial"),
         ("reduce_f32_2p29", lambda aux: mgpu_hip.reduce(xf, rf, aux=aux), 4 * s,
          "mgpu_reduce_partial"),
+        ("segmented_scan_exclusive_i32_2p29_65536_segments",
+         lambda aux: mgpu_hip.segmented_scan(xi, yi, segs, aux=aux), 8 * s, "mgpu_segscan"),
     ]
+    from mgpu_hip import workloads
+    lens = workloads.lognormal_lengths(65536, 2048.0, 1.5, 1, 1 << 22, seed=0)
+    starts = workloads.segment_starts(lens)
+    segs = torch.from_numpy(starts[starts < s]).to(dev)
     for name, fn, nbytes, kname in cases:
         aux = fn(None)
         run = lambda fn=fn, aux=aux: fn(aux)

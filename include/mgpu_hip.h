@@ -140,6 +140,23 @@ MGPU_DECL_SCAN(f32, float)
  * since a 64-bit sum and a look-back flag do not fit one atomic word.      */
 MGPU_DECL_SCAN(u64, uint64_t)
 
+/* Segmented scan (no counterpart in the reference, whose scan has no
+ * segmented form): one scan per segment, each starting from init.
+ * segments[i] (device, int32, ascending) is the first element of segment i;
+ * element 0 always starts a segment; starts at or beyond count are empty.
+ *   exclusive: out[i] = init + in[h] + ... + in[i-1]   (h = its segment start)
+ *   inclusive: out[i] = init + in[h] + ... + in[i]
+ * Single pass: a tile holding a segment start needs no carry beyond it.
+ * input == output is allowed.  count < 2^31.                               */
+#define MGPU_DECL_SEGSCAN(T, TT)                                               \
+  mgpu_status_t mgpu_segmented_scan_##T(void* aux, size_t* aux_size,           \
+      mgpu_stream_t stream, const TT* input, TT* output, int64_t count,        \
+      const int32_t* segments, int64_t num_segments, mgpu_scan_type_t type,    \
+      TT init);
+MGPU_DECL_SEGSCAN(i32, int32_t)
+MGPU_DECL_SEGSCAN(u32, uint32_t)
+MGPU_DECL_SEGSCAN(f32, float)
+
 /* ------------------------------------------------------------------------ */
 /* Batched segmented sort (no counterpart in the reference, which has no
  * segmented primitives; the call shape follows the mgpu family's

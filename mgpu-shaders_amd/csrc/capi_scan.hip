@@ -51,3 +51,20 @@ extern "C" mgpu_status_t mgpu_reduce_u64(void* aux, size_t* aux_size, mgpu_strea
   if (!aux_size) return MGPU_INVALID_ARG;
   return reduce<256>(aux, *aux_size, (hipStream_t)stream, input, count, out, init);
 }
+
+// Segmented scan (no counterpart in the reference).
+#include "mgpu/hip/segscan.hxx"
+
+#define MGPU_DEF_SEGSCAN(T, TT)                                                             \
+  extern "C" mgpu_status_t mgpu_segmented_scan_##T(                                         \
+      void* aux, size_t* aux_size, mgpu_stream_t stream, const TT* input, TT* output,       \
+      int64_t count, const int32_t* segments, int64_t num_segments, mgpu_scan_type_t type,  \
+      TT init) {                                                                            \
+    if (!aux_size) return MGPU_INVALID_ARG;                                                 \
+    return segmented_scan<256, 16>(aux, *aux_size, (hipStream_t)stream, input, output,      \
+                                   count, segments, num_segments, type, init);              \
+  }
+
+MGPU_DEF_SEGSCAN(i32, int32_t)
+MGPU_DEF_SEGSCAN(u32, uint32_t)
+MGPU_DEF_SEGSCAN(f32, float)

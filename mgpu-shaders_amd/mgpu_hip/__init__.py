@@ -88,6 +88,9 @@ def load():
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, vp, i64, c_int, ct]
         f = getattr(lib, "mgpu_reduce_%s" % t)
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, i64, vp, ct]
+    for t in ("i32", "u32", "f32"):
+        f = getattr(lib, "mgpu_segmented_scan_%s" % t)
+        f.restype, f.argtypes = c_int, [vp, sz, vp, vp, vp, i64, vp, i64, c_int, _CTYPE[t]]
     class SegsortTuning(ctypes.Structure):
         _fields_ = [("num_buckets", ctypes.c_int), ("cap", ctypes.c_int * 6),
                     ("shape", ctypes.c_int * 6)]
@@ -294,6 +297,24 @@ def scan_into(input, output, inclusive=False, init=0, elem_type=None, aux=None, 
     return _run_two_phase(fn, [_ptr(input), _ptr(output), input.numel(),
                                INCLUSIVE if inclusive else EXCLUSIVE, _CTYPE[t](init)],
                           "scan", aux, stream)
+
+
+def segmented_scan(input, output, segments, inclusive=False, init=0, elem_type=None, aux=None,
+                   stream=None):
+    """One scan per segment, each from init; segments = int32 device tensor of
+    ascending segment starts (element 0 always starts a segment)."""
+    _device_check(input, output, segments)
+    if output.numel() != input.numel():
+        raise ValueError("output must match input")
+    if segments.dtype != torch.int32:
+        raise ValueError("segments must be int32")
+    t = _scan_type(input, elem_type)
+    if t == "u64":
+        raise ValueError("segmented scan supports int32, uint32 and float32")
+    fn = getattr(load(), "mgpu_segmented_scan_%s" % t)
+    return _run_two_phase(fn, [_ptr(input), _ptr(output), input.numel(), _ptr(segments),
+                               segments.numel(), INCLUSIVE if inclusive else EXCLUSIVE,
+                               _CTYPE[t](init)], "segmented_scan", aux, stream)
 
 
 def reduce(input, out, init=0, elem_type=None, aux=None, stream=None):

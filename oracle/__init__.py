@@ -53,6 +53,8 @@ def lib():
         L.oracle_stable_sort_radix.argtypes = [c_int, vp, vp, i64]
         L.oracle_segmented_stable_sort.argtypes = [c_int, vp, vp, i64, vp, i64, c_int]
         L.oracle_std_sort_u32.argtypes = [vp, i64]
+        L.oracle_segmented_scan_u32.argtypes = [vp, vp, i64, vp, i64, c_int, ctypes.c_uint32]
+        L.oracle_segmented_scan_f64.argtypes = [vp, vp, i64, vp, i64, c_int, ctypes.c_double]
         L.oracle_merge.argtypes = [c_int, vp, vp, i64, vp, vp, i64, vp, vp, c_int]
         for n in ("oracle_exclusive_scan_u32", "oracle_inclusive_scan_u32"):
             getattr(L, n).argtypes = [vp, vp, i64, ctypes.c_uint32]
@@ -181,6 +183,26 @@ def exclusive_scan_f64(x, init=0.0):
     x = np.ascontiguousarray(x, dtype=np.float32)
     out = np.empty(len(x), dtype=np.float64)
     lib().oracle_exclusive_scan_f64(_p(x), _p(out), len(x), init)
+    return out
+
+
+def segmented_scan_u32(x, segments, inclusive=False, init=0):
+    """Per-segment std::exclusive_scan / inclusive_scan mod 2^32 from init."""
+    x = np.ascontiguousarray(x).view(np.uint32)
+    sg = np.ascontiguousarray(segments, dtype=np.int32)
+    out = np.empty_like(x)
+    lib().oracle_segmented_scan_u32(_p(x), _p(out), len(x), _p(sg), len(sg), int(inclusive),
+                                    init & 0xFFFFFFFF)
+    return out
+
+
+def segmented_scan_f64(x, segments, inclusive=False, init=0.0):
+    """Per-segment float64 prefix sums of float32 data from init."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    sg = np.ascontiguousarray(segments, dtype=np.int32)
+    out = np.empty(len(x), dtype=np.float64)
+    lib().oracle_segmented_scan_f64(_p(x), _p(out), len(x), _p(sg), len(sg), int(inclusive),
+                                    float(init))
     return out
 
 

@@ -7,7 +7,8 @@
 // Two kinds of functions:
 //   * standard oracles: std::stable_sort, std::merge, std::exclusive_scan /
 //     std::inclusive_scan in uint32 modular arithmetic, float64 prefix sums,
-//     std::reduce (the reference's own CPU comparators: std::sort in
+//     std::reduce, and their per-segment forms for the segmented sort and
+//     scan (the reference's own CPU comparators: std::sort in
 //     examples/vk_radix2/vk_radix2.cxx:32, std::sort before merging in
 //     examples/vk_merge/vk_merge.cxx:26-27);
 //   * restatements of the reference's algorithms, run sequentially on the CPU:
@@ -493,6 +494,36 @@ void oracle_inclusive_scan_f64(const float* in, double* out, int64_t n, double i
 void oracle_exclusive_scan_f64(const float* in, double* out, int64_t n, double init) {
   double run = init;
   for (int64_t i = 0; i < n; ++i) { out[i] = run; run += (double)in[i]; }
+}
+// Segmented scans: std::exclusive_scan / std::inclusive_scan from init over
+// every segment [segs[i], segs[i+1]) (the last ends at n); element 0 always
+// starts a segment, so [0, segs[0]) is one too.
+}  // extern "C"
+template <typename F>
+static void for_each_segment(int64_t n, const int32_t* segs, int64_t num_segments, F f) {
+  int64_t b = 0;
+  for (int64_t i = 0; i <= num_segments; ++i) {
+    int64_t e = i < num_segments ? std::min<int64_t>(std::max<int64_t>(segs[i], 0), n) : n;
+    if (e > b) {
+      f(b, e);
+      b = e;
+    }
+  }
+}
+extern "C" {
+void oracle_segmented_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, const int32_t* segs,
+                               int64_t num_segments, int inclusive, uint32_t init) {
+  for_each_segment(n, segs, num_segments, [&](int64_t b, int64_t e) {
+    if (inclusive) std::inclusive_scan(in + b, in + e, out + b, std::plus<uint32_t>(), init);
+    else std::exclusive_scan(in + b, in + e, out + b, init);
+  });
+}
+void oracle_segmented_scan_f64(const float* in, double* out, int64_t n, const int32_t* segs,
+                               int64_t num_segments, int inclusive, double init) {
+  for_each_segment(n, segs, num_segments, [&](int64_t b, int64_t e) {
+    if (inclusive) oracle_inclusive_scan_f64(in + b, out + b, e - b, init);
+    else oracle_exclusive_scan_f64(in + b, out + b, e - b, init);
+  });
 }
 // std::inclusive_scan in float32 (the CPU baseline as timed).
 void oracle_inclusive_scan_f32(const float* in, float* out, int64_t n) {

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
+from mgpu_hip import workloads
 
 KTS = ("u32", "i32", "u64", "f32")
 
@@ -207,6 +208,60 @@ def test_vk_scan_known_answer():
     exp[0] = 0
     np.testing.assert_array_equal(oracle.exclusive_scan_u32(i), exp)
     np.testing.assert_array_equal(oracle.scan_schedule_u32(i, 896), exp)
+
+
+def np_segmented_scan(x, segs, inclusive, init, dtype):
+    """Segmented scan by differences of one global cumsum (an algorithm
+    unlike the oracle's per-segment std::scan)."""
+    n = len(x)
+    heads = np.zeros(n, dtype=bool)
+    heads[0] = True
+    s = np.asarray(segs, dtype=np.int64)
+    heads[s[(s >= 0) & (s < n)]] = True
+    seg_id = np.cumsum(heads) - 1
+    start = np.flatnonzero(heads)
+    c = np.concatenate([np.zeros(1, dtype), np.cumsum(x.astype(dtype), dtype=dtype)])
+    first = start[seg_id]
+    upto = np.arange(n) + (1 if inclusive else 0)
+    return c[upto] - c[first] + dtype(init)
+
+
+@pytest.mark.parametrize("inclusive", [False, True])
+def test_segmented_scan_vs_numpy(inclusive):
+    n = 50000
+    x = oracle.generate(n, "u32", 8)
+    lens = workloads.lognormal_lengths(400, 64.0, 1.5, 0, 4000, seed=3)
+    segs = workloads.segment_starts(lens)
+    segs = segs[segs <= n]
+    segs = np.concatenate([segs, [n, n, n + 5]]).astype(np.int32)  # empty trailing segments
+    with np.errstate(over="ignore"):
+        exp = np_segmented_scan(x, segs, inclusive, 11, np.uint64) & np.uint64(0xFFFFFFFF)
+    np.testing.assert_array_equal(oracle.segmented_scan_u32(x, segs, inclusive, 11), exp)
+    f = oracle.generate(n, "f32_01", 9)
+    ints = (f.astype(np.float64) * 2 ** 24).astype(np.int64)
+    expf = np_segmented_scan(ints, segs, inclusive, 0, np.int64).astype(np.float64) * 2.0 ** -24
+    np.testing.assert_array_equal(oracle.segmented_scan_f64(f, segs, inclusive, 0.0), expf)
+
+
+def test_segmented_scan_special_cases():
+    x = oracle.generate(3000, "i32_0_16", 2)
+    # no segments, or one at 0: the plain scan
+    for segs in (np.zeros(0, np.int32), np.zeros(1, np.int32)):
+        np.testing.assert_array_equal(oracle.segmented_scan_u32(x, segs, False, 5),
+                                      oracle.exclusive_scan_u32(x, 5))
+        np.testing.assert_array_equal(oracle.segmented_scan_u32(x, segs, True, 5),
+                                      oracle.inclusive_scan_u32(x, 5))
+    # every element a segment: exclusive = init, inclusive = init + x
+    every = np.arange(3000, dtype=np.int32)
+    assert np.all(oracle.segmented_scan_u32(x, every, False, 5) == 5)
+    np.testing.assert_array_equal(oracle.segmented_scan_u32(x, every, True, 5),
+                                  x.view(np.uint32) + np.uint32(5))
+    # duplicate starts (empty segments) change nothing
+    segs = np.array([0, 100, 100, 100, 2000], dtype=np.int32)
+    np.testing.assert_array_equal(oracle.segmented_scan_u32(x, segs, False, 0),
+                                  oracle.segmented_scan_u32(x, np.array([0, 100, 2000], np.int32), False, 0))
+    # a segment restarts the sum: element 100 of an exclusive scan is init
+    assert oracle.segmented_scan_u32(x, segs, False, 3)[100] == 3
 
 
 # --------------------------------------------------------- restatements
