@@ -100,7 +100,58 @@ def cpu_baseline(log2n):
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "keys/s", "cores": 1, "kind": "port",
             "sample": "std::stable_sort of 2^%d (u32 key, u32 index) pairs, uniform keys "
-                      "(seed 0), %.2f s on one core, nproc=%d" % (log2n, dt, os.cpu_count() or 0)}
+                      "(seed 0), %.2f s on one core, nproc=%d" % (log2n, dt, os.cpu_count() or 0),
+            "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines_other(log2n=27):
+    """The reference's other CPU comparators on one core (the oracle):
+    std::merge, std::exclusive_scan, std::inclusive_scan (float32) and
+    std::reduce over 2^log2n elements, median of 3."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    n = 1 << log2n
+
+    def med(fn):
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[1]
+
+    import numpy as np
+    a = np.sort(oracle.generate(n // 2, "u32", 1))
+    b = np.sort(oracle.generate(n // 2, "u32", 2))
+    xi = oracle.generate(n, "i32_0_16", 0)
+    xf = oracle.generate(n, "f32_01", 0)
+    out = {}
+    dt = med(lambda: oracle.merge(a, b))
+    out["std_merge_u32"] = {"GBps": 8 * n / dt / 1e9, "keys_per_s": n / dt, "s": dt,
+                            "sample": "2 x 2^%d sorted u32 keys" % (log2n - 1)}
+    dt = med(lambda: oracle.exclusive_scan_u32(xi))
+    out["std_exclusive_scan_i32"] = {"GBps": 8 * n / dt / 1e9, "s": dt,
+                                     "sample": "2^%d int32 in [0, 16)" % log2n}
+    dt = med(lambda: oracle.inclusive_scan_f32(xf))
+    out["std_inclusive_scan_f32"] = {"GBps": 8 * n / dt / 1e9, "s": dt,
+                                     "sample": "2^%d float32 in [0, 1)" % log2n}
+    dt = med(lambda: oracle.reduce_u32(xi))
+    out["std_reduce_i32"] = {"GBps": 4 * n / dt / 1e9, "s": dt,
+                             "sample": "2^%d int32 in [0, 16)" % log2n}
+    out["cores"] = 1
+    out["kind"] = "port"
+    return out
 
 
 def secondary(torch, mgpu_hip, steps, warmup):
@@ -406,6 +457,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline 2^%d" % args.cpu_log2)
         result["cpu_baseline"] = cpu_baseline(args.cpu_log2)
+        if not args.no_secondary:
+            result["cpu_baselines_other"] = cpu_baselines_other()
     print(json.dumps(result), flush=True)
 
 

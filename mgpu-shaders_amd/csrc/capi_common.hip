@@ -71,11 +71,7 @@ mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int 
   if ((kind == MGPU_GEN_U64 || kind == MGPU_GEN_SORTED_U64) && elem_bytes != 8) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
   if (!out) return MGPU_INVALID_ARG;
-  int64_t blocks = min_<int64_t>(div_up(count, (int64_t)256), 8192);
-  hipError_t e = launch_kernel("mgpu_generate", kernel_generate<256>, dim3((unsigned)blocks),
-                               dim3(256), 0, (hipStream_t)stream, out, count, elem_bytes,
-                               (int)kind, seed, offset);
-  return hip_status(e);
+  return generate(out, count, elem_bytes, (int)kind, seed, offset, (hipStream_t)stream);
 }
 
 void mgpu_set_lookback_spin_limit(uint32_t limit) {

@@ -108,10 +108,7 @@ size_t request_bytes(const mgpu_request_t& r) {
 }
 
 hipError_t gen(hipStream_t s, void* out, int64_t n, int eb, int kind, uint64_t seed) {
-  if (n <= 0) return hipSuccess;
-  int64_t blocks = min_<int64_t>(div_up(n, (int64_t)256), 8192);
-  return launch_kernel("mgpu_generate", kernel_generate<256>, dim3((unsigned)blocks), dim3(256), 0,
-                       s, out, n, eb, kind, seed, (uint64_t)0);
+  return generate(out, n, eb, kind, seed, 0, s) == MGPU_OK ? hipSuccess : hipErrorLaunchFailure;
 }
 
 hipError_t alloc(rec_t* r, void** p, size_t bytes, hipStream_t s) {

@@ -15,7 +15,7 @@
 //   sorted_f32  (i + (u32(0, i) >> 8) * 2^-24) / n               (ascending)
 // (sorted kinds use n = the generated count and ignore the offset)
 #pragma once
-#include "meta.hxx"
+#include "transform.hxx"
 
 namespace mgpu {
 namespace hip {
@@ -80,16 +80,22 @@ __host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_
   }
 }
 
-template <int NT>
-__global__ __launch_bounds__(NT) void kernel_generate(void* out, int64_t count, int elem_bytes,
-                                                      int kind, uint64_t seed, uint64_t offset) {
-  int64_t stride = (int64_t)gridDim.x * NT;
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < count; i += stride) {
-    uint64_t v = kind >= gen_sorted_u32 ? gen_element(kind, seed, (uint64_t)i, (uint64_t)count)
-                                        : gen_element(kind, seed, offset + (uint64_t)i);
-    if (elem_bytes == 8) ((uint64_t*)out)[i] = v;
-    else ((uint32_t*)out)[i] = (uint32_t)v;
+// Fill out[0, count) with a generator kind (4- or 8-byte elements), one
+// element per thread through transform().
+inline mgpu_status_t generate(void* out, int64_t count, int elem_bytes, int kind, uint64_t seed,
+                              uint64_t offset, hipStream_t stream) {
+  if (elem_bytes == 8) {
+    uint64_t* o = (uint64_t*)out;
+    return transform<256>(count, stream, [=] __device__(int64_t i) {
+      o[i] = kind >= gen_sorted_u32 ? gen_element(kind, seed, (uint64_t)i, (uint64_t)count)
+                                    : gen_element(kind, seed, offset + (uint64_t)i);
+    }, "mgpu_generate");
   }
+  uint32_t* o = (uint32_t*)out;
+  return transform<256>(count, stream, [=] __device__(int64_t i) {
+    o[i] = (uint32_t)(kind >= gen_sorted_u32 ? gen_element(kind, seed, (uint64_t)i, (uint64_t)count)
+                                             : gen_element(kind, seed, offset + (uint64_t)i));
+  }, "mgpu_generate");
 }
 
 }  // namespace hip

@@ -1,0 +1,199 @@
+// C++ consumer of the header-template surface (mgpu/hip/*.hxx): user
+// comparator functors through mergesort and merge, launch / transform with
+// device lambdas, and transform_scan with a user functor.  Each check is
+// against the std:: algorithm on the host.  Prints "[case] <name>" before
+// each GPU step and "PASS <name>"; exits non-zero on the first mismatch.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <numeric>
+#include <vector>
+
+#include "mgpu/hip/generate.hxx"
+#include "mgpu/hip/merge.hxx"
+#include "mgpu/hip/mergesort.hxx"
+#include "mgpu/hip/scan.hxx"
+#include "mgpu/hip/transform.hxx"
+
+using namespace mgpu::hip;
+
+// Descending by the low byte only: 256 classes, so stability is visible.
+struct low_byte_desc_t {
+  __host__ __device__ bool operator()(uint32_t a, uint32_t b) const {
+    return (a & 255u) > (b & 255u);
+  }
+};
+// Ascending by magnitude: -x and x tie.
+struct abs_less_t {
+  __host__ __device__ bool operator()(float a, float b) const { return fabsf(a) < fabsf(b); }
+};
+struct low_nibble_t {
+  __host__ __device__ uint32_t operator()(uint32_t x) const { return x & 15u; }
+};
+
+#define CHECK_HIP(x)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) {                                                 \
+      std::printf("FAIL hip %s at line %d\n", hipGetErrorString(e_), __LINE__); \
+      return 1;                                                             \
+    }                                                                       \
+  } while (0)
+#define CHECK_OK(...)                                                       \
+  do {                                                                      \
+    mgpu_status_t s_ = (__VA_ARGS__);                                       \
+    if (s_ != MGPU_OK) {                                                    \
+      std::printf("FAIL status %d at line %d\n", (int)s_, __LINE__);       \
+      return 1;                                                             \
+    }                                                                       \
+  } while (0)
+
+static void banner(const char* name) {
+  std::printf("[case] %s\n", name);
+  std::fflush(stdout);
+}
+
+template <typename T>
+static std::vector<T> to_host(const T* d, size_t n) {
+  std::vector<T> h(n);
+  if (n && hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) h.clear();
+  return h;
+}
+
+int main() {
+  hipStream_t stream = nullptr;
+  const int64_t n = (1 << 20) + 17;
+
+  {
+    banner("transform and launch");
+    int64_t *a = nullptr, *b = nullptr;
+    const int blocks = 100;
+    CHECK_HIP(hipMalloc(&a, n * sizeof(int64_t)));
+    CHECK_HIP(hipMalloc(&b, blocks * 256 * sizeof(int64_t)));
+    CHECK_OK(transform(n, stream, [=] __device__(int64_t i) { a[i] = 3 * i + 1; }));
+    CHECK_OK(launch<256>(blocks, stream, [=] __device__(int tid, int cta) {
+      b[cta * 256 + tid] = (int64_t)cta * 1000 + tid;
+    }));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<int64_t> ha = to_host(a, n), hb = to_host(b, blocks * 256);
+    for (int64_t i = 0; i < n; ++i)
+      if (ha[i] != 3 * i + 1) { std::printf("FAIL transform at %ld\n", (long)i); return 1; }
+    for (int c = 0; c < blocks; ++c)
+      for (int t = 0; t < 256; ++t)
+        if (hb[c * 256 + t] != (int64_t)c * 1000 + t) { std::printf("FAIL launch\n"); return 1; }
+    CHECK_HIP(hipFree(a));
+    CHECK_HIP(hipFree(b));
+    std::printf("PASS transform and launch\n");
+  }
+
+  {
+    banner("mergesort_kv with a user comparator");
+    uint32_t *keys = nullptr, *vals = nullptr;
+    CHECK_HIP(hipMalloc(&keys, n * 4));
+    CHECK_HIP(hipMalloc(&vals, n * 4));
+    CHECK_OK(generate(keys, n, 4, gen_u32, 11, 0, stream));
+    CHECK_OK(generate(vals, n, 4, gen_iota, 0, 0, stream));
+    std::vector<uint32_t> hk = to_host(keys, n);
+    size_t aux_size = 0;
+    CHECK_OK(mergesort_kv<256, 15>(nullptr, aux_size, stream, keys, vals, n, low_byte_desc_t()));
+    void* aux = nullptr;
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(mergesort_kv<256, 15>(aux, aux_size, stream, keys, vals, n, low_byte_desc_t()));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](uint32_t x, uint32_t y) { return low_byte_desc_t()(hk[x], hk[y]); });
+    std::vector<uint32_t> gk = to_host(keys, n), gv = to_host(vals, n);
+    for (int64_t i = 0; i < n; ++i)
+      if (gv[i] != idx[i] || gk[i] != hk[idx[i]]) {
+        std::printf("FAIL mergesort at %ld\n", (long)i);
+        return 1;
+      }
+    CHECK_HIP(hipFree(aux));
+    CHECK_HIP(hipFree(keys));
+    CHECK_HIP(hipFree(vals));
+    std::printf("PASS mergesort_kv with a user comparator\n");
+  }
+
+  {
+    banner("merge with a user comparator");
+    const int64_t na = n, nb = n / 3 + 5;
+    std::vector<float> ha(na), hb(nb);
+    float *a = nullptr, *b = nullptr, *c = nullptr;
+    uint32_t *av = nullptr, *bv = nullptr, *cv = nullptr;
+    CHECK_HIP(hipMalloc(&a, na * 4));
+    CHECK_HIP(hipMalloc(&b, nb * 4));
+    CHECK_OK(generate(a, na, 4, gen_f32_pm1, 21, 0, stream));
+    CHECK_OK(generate(b, nb, 4, gen_f32_pm1, 22, 0, stream));
+    CHECK_HIP(hipDeviceSynchronize());
+    ha = to_host(a, na);
+    hb = to_host(b, nb);
+    // coarse magnitudes so that many keys tie under the comparator
+    for (float& x : ha) x = std::round(x * 64.0f) / 64.0f;
+    for (float& x : hb) x = std::round(x * 64.0f) / 64.0f;
+    std::stable_sort(ha.begin(), ha.end(), abs_less_t());
+    std::stable_sort(hb.begin(), hb.end(), abs_less_t());
+    CHECK_HIP(hipMemcpy(a, ha.data(), na * 4, hipMemcpyHostToDevice));
+    CHECK_HIP(hipMemcpy(b, hb.data(), nb * 4, hipMemcpyHostToDevice));
+    CHECK_HIP(hipMalloc(&av, na * 4));
+    CHECK_HIP(hipMalloc(&bv, nb * 4));
+    CHECK_HIP(hipMalloc(&c, (na + nb) * 4));
+    CHECK_HIP(hipMalloc(&cv, (na + nb) * 4));
+    CHECK_OK(transform(na, stream, [=] __device__(int64_t i) { av[i] = (uint32_t)i; }));
+    CHECK_OK(transform(nb, stream, [=] __device__(int64_t i) { bv[i] = (uint32_t)(na + i); }));
+    size_t aux_size = 0;
+    CHECK_OK(merge<256, 15>(nullptr, aux_size, stream, (const float*)a, (const uint32_t*)av, na,
+                            (const float*)b, (const uint32_t*)bv, nb, c, cv, abs_less_t()));
+    void* aux = nullptr;
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(merge<256, 15>(aux, aux_size, stream, (const float*)a, (const uint32_t*)av, na,
+                            (const float*)b, (const uint32_t*)bv, nb, c, cv, abs_less_t()));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<std::pair<float, uint32_t>> pa(na), pb(nb), pc(na + nb);
+    for (int64_t i = 0; i < na; ++i) pa[i] = {ha[i], (uint32_t)i};
+    for (int64_t i = 0; i < nb; ++i) pb[i] = {hb[i], (uint32_t)(na + i)};
+    std::merge(pa.begin(), pa.end(), pb.begin(), pb.end(), pc.begin(),
+               [](const std::pair<float, uint32_t>& x, const std::pair<float, uint32_t>& y) {
+                 return abs_less_t()(x.first, y.first);
+               });
+    std::vector<float> gc = to_host(c, na + nb);
+    std::vector<uint32_t> gcv = to_host(cv, na + nb);
+    for (int64_t i = 0; i < na + nb; ++i)
+      if (gcv[i] != pc[i].second || gc[i] != pc[i].first) {
+        std::printf("FAIL merge at %ld\n", (long)i);
+        return 1;
+      }
+    for (void* p : {(void*)a, (void*)b, (void*)c, (void*)av, (void*)bv, (void*)cv, aux})
+      CHECK_HIP(hipFree(p));
+    std::printf("PASS merge with a user comparator\n");
+  }
+
+  {
+    banner("transform_scan with a user functor");
+    uint32_t *x = nullptr, *y = nullptr;
+    CHECK_HIP(hipMalloc(&x, n * 4));
+    CHECK_HIP(hipMalloc(&y, n * 4));
+    CHECK_OK(generate(x, n, 4, gen_u32, 31, 0, stream));
+    size_t aux_size = 0;
+    CHECK_OK(transform_scan<256, 16>(nullptr, aux_size, stream, (const uint32_t*)x, y, n,
+                                     MGPU_SCAN_EXCLUSIVE, 5u, low_nibble_t()));
+    void* aux = nullptr;
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(transform_scan<256, 16>(aux, aux_size, stream, (const uint32_t*)x, y, n,
+                                     MGPU_SCAN_EXCLUSIVE, 5u, low_nibble_t()));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<uint32_t> hx = to_host(x, n), hy = to_host(y, n);
+    uint32_t run = 5;
+    for (int64_t i = 0; i < n; ++i) {
+      if (hy[i] != run) { std::printf("FAIL transform_scan at %ld\n", (long)i); return 1; }
+      run += hx[i] & 15u;
+    }
+    CHECK_HIP(hipFree(aux));
+    CHECK_HIP(hipFree(x));
+    CHECK_HIP(hipFree(y));
+    std::printf("PASS transform_scan with a user functor\n");
+  }
+  std::printf("ALL PASS\n");
+  return 0;
+}
