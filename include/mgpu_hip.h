@@ -121,8 +121,9 @@ MGPU_DECL_RADIX(f32, float)
  *   applied once (std::exclusive_scan semantics).
  * mgpu_scan_T: out of place or in place (input == output), exclusive or
  *   inclusive.
- * Integer types wrap modulo 2^32 (two's complement).  f32: hierarchical
- *   float32 sums inside a 4096-element tile, float64 carries between tiles.
+ * Integer types wrap modulo 2^32 (two's complement).  f32: float32 sums
+ *   inside aligned 256-element blocks (at most 10 additions on any path),
+ *   float64 for every sum above a block and for the carries between tiles.
  * mgpu_reduce_T: *out (device) := init + sum(input[0..count)); f32 sums in
  *   float64 and rounds once.                                                */
 #define MGPU_DECL_SCAN(T, TT)                                                  \

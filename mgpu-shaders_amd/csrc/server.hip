@@ -100,7 +100,7 @@ size_t request_bytes(const mgpu_request_t& r) {
       if (r.key_type == MGPU_KEY_U64)
         scan64<uint64_t>(nullptr, aux, nullptr, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
       else
-        scan<256, 16>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
+        scan<scan_tuning::nt, scan_tuning::vt>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
                       MGPU_SCAN_EXCLUSIVE, 0u);
       return 2 * aux_round(n * kb) + aux_round(aux);
   }
@@ -199,7 +199,7 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
       if (q.key_type == MGPU_KEY_U64)
         scan64<uint64_t>(nullptr, aux, s, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
       else
-        scan<256, 16>(nullptr, aux, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
+        scan<scan_tuning::nt, scan_tuning::vt>(nullptr, aux, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
                       MGPU_SCAN_EXCLUSIVE, 0u);
       if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->out, n * kb, s)) ||
           !ok(alloc(r, &r->aux, aux, s)))
@@ -212,10 +212,10 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
           return scan64<uint64_t>(r->aux, cap, s, (const uint64_t*)r->in_a, (uint64_t*)r->out, n,
                                   MGPU_SCAN_EXCLUSIVE, 0ull);
         case MGPU_KEY_F32:
-          return scan<256, 16>(r->aux, cap, s, (const float*)r->in_a, (float*)r->out, n,
+          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, cap, s, (const float*)r->in_a, (float*)r->out, n,
                                MGPU_SCAN_INCLUSIVE, 0.0f);
         default:
-          return scan<256, 16>(r->aux, cap, s, (const uint32_t*)r->in_a, (uint32_t*)r->out, n,
+          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, cap, s, (const uint32_t*)r->in_a, (uint32_t*)r->out, n,
                                MGPU_SCAN_EXCLUSIVE, 0u);
       }
     }

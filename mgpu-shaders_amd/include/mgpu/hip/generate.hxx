@@ -59,7 +59,7 @@ __host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_
     }
     case gen_sorted_f32: {
       float f = ((float)i + (float)(gen_word32(seed, 0, i) >> 8) * (1.0f / 16777216.0f)) / (float)n;
-      uint32_t b; std::memcpy(&b, &f, 4); return b;
+      uint32_t b; __builtin_memcpy(&b, &f, 4); return b;
     }
     case gen_u32: return gen_word32(seed, 0, i);
     case gen_lowentropy:
@@ -68,12 +68,12 @@ __host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_
     case gen_i32_0_16: return gen_word32(seed, 0, i) >> 28;
     case gen_f32_01: {
       float f = (float)(gen_word32(seed, 0, i) >> 8) * (1.0f / 16777216.0f);
-      uint32_t b; std::memcpy(&b, &f, 4); return b;
+      uint32_t b; __builtin_memcpy(&b, &f, 4); return b;
     }
     case gen_u64: return ((uint64_t)gen_word32(seed, 0, i) << 32) | gen_word32(seed, 1, i);
     case gen_f32_pm1: {
       float f = (float)(gen_word32(seed, 0, i) >> 8) * (1.0f / 8388608.0f) - 1.0f;
-      uint32_t b; std::memcpy(&b, &f, 4); return b;
+      uint32_t b; __builtin_memcpy(&b, &f, 4); return b;
     }
     case gen_iota: return i;
     default: return 0;

@@ -73,7 +73,7 @@ struct radix_permute_t {
 
   __host__ __device__ static unsigned_type to_radix_bits(key_t x) {
     unsigned_type y;
-    std::memcpy(&y, &x, sizeof(y));
+    __builtin_memcpy(&y, &x, sizeof(y));
     if constexpr (std::is_unsigned_v<key_t>) {
       return y;
     } else if constexpr (std::is_integral_v<key_t>) {
@@ -85,15 +85,15 @@ struct radix_permute_t {
   }
   __host__ __device__ static key_t from_radix_bits(unsigned_type y) {
     if constexpr (std::is_unsigned_v<key_t>) {
-      key_t x; std::memcpy(&x, &y, sizeof(x)); return x;
+      key_t x; __builtin_memcpy(&x, &y, sizeof(x)); return x;
     } else if constexpr (std::is_integral_v<key_t>) {
       y ^= msb;
-      key_t x; std::memcpy(&x, &y, sizeof(x)); return x;
+      key_t x; __builtin_memcpy(&x, &y, sizeof(x)); return x;
     } else {
       y ^= msb;
       unsigned_type mask = (unsigned_type)((signed_type)y >> (8 * sizeof(key_t) - 1)) & (msb - 1);
       y ^= mask;
-      key_t x; std::memcpy(&x, &y, sizeof(x)); return x;
+      key_t x; __builtin_memcpy(&x, &y, sizeof(x)); return x;
     }
   }
 };

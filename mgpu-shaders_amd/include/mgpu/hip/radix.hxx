@@ -83,7 +83,7 @@ __device__ inline void cta_radix_count(const key_t* keys, int tile_count, int sh
 #pragma unroll
     for (int i = 0; i < NVEC / CNT; ++i) {
       key_t kk[KPV];
-      std::memcpy(kk, &v[i], 16);
+      __builtin_memcpy(kk, &v[i], 16);
 #pragma unroll
       for (int e = 0; e < KPV; ++e) add(kk[e]);
     }

@@ -11,8 +11,16 @@
 // Summation order inside a tile is hierarchical as in the reference
 // (thread-serial over 4 consecutive elements, cta_scan.hxx:150; a wave scan,
 // :100-104; the wave totals in order, :107-130; then the carry-in, :157-171).
-// Between tiles the carry is float64 for float32 data; for integers all
-// arithmetic is modulo 2^32.
+// A tile is NT x VT elements in rows of 4 * NT; each wave scans an aligned
+// block of 256 consecutive elements of a row in the element type, and every
+// sum above such a block (earlier blocks of the tile, the carry between
+// tiles) is taken in the carry type: float64 for float32 data, so a float32
+// result has at most 10 float32 additions on its path (3 thread-serial,
+// 6 wave-scan levels, the wave prefix) and does not depend on the tile shape.
+// For integers all arithmetic is modulo 2^32.  Tiles are large (20480
+// elements by default) because each tile's look-back costs a few memory
+// round trips: fewer tiles per byte is what moves the scan toward the copy
+// bandwidth.
 #pragma once
 #include "meta.hxx"
 
@@ -131,7 +139,7 @@ __device__ inline void scan_load_tile(const in_t* input, int64_t count, int64_t 
     for (int r = 0; r < NR; ++r) {
       u32x4 v = __builtin_nontemporal_load((const u32x4*)(input + base) + (r * NT + tid));
       in_t e[4];
-      std::memcpy(e, &v, 16);
+      __builtin_memcpy(e, &v, 16);
 #pragma unroll
       for (int q = 0; q < 4; ++q) x[4 * r + q] = (acc_t)(T)xform(e[q]);
     }
@@ -187,20 +195,14 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
   }
   __syncthreads();
 
-  acc_t run = 0, off[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    acc_t pre = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      acc_t v = s_wave[r][w];
-      if (w < warp) pre += v;
-      tot += v;
-    }
-    off[r] = run + pre;
-    run += tot;
+  // Tile aggregate in the carry type (float64 for float32 data), summed by
+  // wave 0 over the NR x NW wave totals.
+  carry_t aggregate = 0;
+  if (warp == 0) {
+    const acc_t* flat = &s_wave[0][0];
+    for (int i = lane; i < NR * NW; i += warp_size) aggregate += (carry_t)flat[i];
+    aggregate = wave_reduce_all(aggregate);
   }
-  const acc_t aggregate = run;
 
   if (warp == 0) {
     carry_t carry;
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
       carry = (carry_t)(acc_t)init;
     } else {
       if (lane == 0) {
-        atomic_store_agent(status + tile, pack_status((carry_t)aggregate, status_aggregate));
+        atomic_store_agent(status + tile, pack_status(aggregate, status_aggregate));
         // The aggregate must be visible before this tile's output (possibly
         // over its own input) can be: wait for the store to complete.
         __builtin_amdgcn_s_waitcnt(0);
@@ -218,22 +220,35 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
     }
     if (lane == 0) {
       atomic_store_agent(status + tile,
-                         pack_status((carry_t)(carry + (carry_t)aggregate), status_inclusive));
+                         pack_status((carry_t)(carry + aggregate), status_inclusive));
       s_carry = carry;
     }
   }
   __syncthreads();
   const carry_t carry = s_carry;
 
+  // Everything above one wave row (the carry, earlier rows, earlier waves of
+  // this row) is summed in the carry type; inside a row the sums are acc_t:
+  // at most 3 thread-serial additions, 6 wave-scan levels and the wave's
+  // exclusive prefix.
   T o[VT];
+  carry_t run = carry;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    acc_t p = off[r] + w_excl[r];
+    carry_t pre = run, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const carry_t v = (carry_t)s_wave[r][w];
+      if (w < warp) pre += v;
+      tot += v;
+    }
+    run += tot;
+    const acc_t p = w_excl[r];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       acc_t local = inclusive ? (acc_t)(p + x[4 * r + q])
                               : (q == 0 ? p : (acc_t)(p + x[4 * r + q - 1]));
-      o[4 * r + q] = (T)carry_combine(carry, local);
+      o[4 * r + q] = (T)carry_combine(pre, local);
     }
   }
   if (vec_ok && count - base >= NV) {
@@ -241,7 +256,7 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       u32x4 v;
-      std::memcpy(&v, &o[4 * r], 16);
+      __builtin_memcpy(&v, &o[4 * r], 16);
       __builtin_nontemporal_store(v, (u32x4*)(output + base) + (r * NT + tid));
     }
   } else {
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
 }
 
 // transform_scan: scan of xform(input[i]).  input/output may alias (in place).
-template <int NT = 256, int VT = 16, typename in_t, typename T, typename xform_t>
+template <int NT = 256, int VT = 80, typename in_t, typename T, typename xform_t>
 mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
                              const in_t* input, T* output, int64_t count,
                              mgpu_scan_type_t type, T init, xform_t xform) {
@@ -290,7 +305,7 @@ mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t strea
   return hip_status(e);
 }
 
-template <int NT = 256, int VT = 16, typename T>
+template <int NT = 256, int VT = 80, typename T>
 mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
                    T* output, int64_t count, mgpu_scan_type_t type = MGPU_SCAN_EXCLUSIVE,
                    T init = T()) {
@@ -300,7 +315,7 @@ mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, const T
 
 // In-place exclusive scan with the reference's argument order
 // (inc/mgpu/vk/scan.hxx:10-13).
-template <int NT = 256, int VT = 16, typename T>
+template <int NT = 256, int VT = 80, typename T>
 mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, T* data,
                    int64_t count, T init = T()) {
   return scan<NT, VT>(aux_data, aux_size, stream, (const T*)data, data, count,
@@ -347,7 +362,7 @@ __global__ __launch_bounds__(NT) void kernel_reduce_partial(
   int64_t i = (int64_t)blockIdx.x * NT + tid;
   auto addv = [&](u32x4 v) {
     T e[EPV];
-    std::memcpy(e, &v, 16);
+    __builtin_memcpy(e, &v, 16);
 #pragma unroll
     for (int q = 0; q < EPV; ++q) sum += (acc_t)e[q];
   };

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       u32x4 v;
-      std::memcpy(&v, &o[4 * r], 16);
+      __builtin_memcpy(&v, &o[4 * r], 16);
       __builtin_nontemporal_store(v, (u32x4*)(output + base) + (r * NT + tid));
     }
   } else {

@@ -80,7 +80,7 @@ def test_sizing_radix_and_scan():
     scan_aux = r256(8 * (-(-tiles * 256 // 4096)))
     assert sz == r256(4 * n) * 2 + r256(4 * tiles * 256) + scan_aux
     st, sz = _size(lib.mgpu_scan_exclusive_i32, ctypes.c_void_p(1), n, 0)
-    assert st == 0 and sz == r256(8 * (n // 4096))
+    assert st == 0 and sz == r256(8 * -(-n // (256 * 80)))  # 8 B per 20480-element tile
 
 
 def test_invalid_arguments():

@@ -11,13 +11,16 @@ from gpu_util import banner, dev_equal, gen_dev, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [1, 2, 3, 4, 5, 63, 64, 65, 4095, 4096, 4097, 3 * 4096 + 7, 64 * 4096 + 1, 1 << 20,
+NV = 256 * 80  # elements per tile of the library's scan
+SIZES = [1, 2, 3, 4, 5, 63, 64, 65, 255, 256, 257, 4095, 4096, 4097, NV - 1, NV, NV + 1,
+         3 * NV + 7, 64 * NV + 1, 1 << 20,
          (1 << 22) + 13]
 
-# float32 scan contract: float32 hierarchical sums inside a 4096-element tile
-# (depth <= 4 + 6 + 2 + 4 = 16 additions on any path), float64 carries
-# between tiles, one rounding at the output.  For non-negative inputs the
-# relative error is below gamma_16 + u ~ 1.0e-6; the bound below is 2e-6.
+# float32 scan contract: float32 sums only inside aligned 256-element blocks
+# (at most 3 thread-serial additions + 6 wave-scan levels + the wave prefix
+# = 10 on any path); every sum above a block and the carries between tiles
+# are float64, and one rounding at the output.  For non-negative inputs the
+# relative error is below gamma_10 + u ~ 6.6e-7; the bound below is 2e-6.
 F32_SCAN_RTOL = 2e-6
 
 
@@ -46,7 +49,7 @@ def test_scan_u32_out_of_place_with_init(n, inclusive):
 @pytest.mark.parametrize("offset", [1, 2, 3])
 def test_scan_misaligned_pointers(offset):
     banner("scan misaligned", offset)
-    n = 3 * 4096 + 5
+    n = 3 * NV + 5
     x = oracle.generate(n + offset, "i32_0_16", 3)
     d = to_dev(x)
     view = d[offset:]
@@ -133,7 +136,7 @@ def test_scan_forced_lookback_fallback(limit, inplace):
     """A tiny spin limit makes tiles sum late predecessors from their input
     (in place: with the re-check of the published word); results unchanged."""
     banner("scan fallback", limit, inplace)
-    n = 300 * 4096 + 17
+    n = 300 * NV + 17
     x = oracle.generate(n, "i32_0_16", 21)
     f = oracle.generate(n, "f32_01", 22)
     old = mgpu_hip.lookback_spin_limit()
