@@ -433,7 +433,7 @@ def main():
         "config": {"workload": "2^28 uint32 keys uniform over [0,2^32), uint32 values = index, "
                                "stable LSD radix sort, 8-bit digits, 4 passes of "
                                "count + scan + scatter",
-                   "n": n, "key": "u32", "value": "u32", "nt": 512, "vt": 24},
+                   "n": n, "key": "u32", "value": "u32", "nt": 512, "vt": 20},
         "roofline": {"bound": "hbm", "kernel": "mgpu_radix_scatter",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

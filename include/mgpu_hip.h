@@ -103,7 +103,7 @@ MGPU_DECL_MERGE(f32, float)
  * value order; signed: MSB flipped; float: -0.0 before +0.0, NaNs last),
  * stable, in place.  Replaces vk::radix_sort (inc/mgpu/vk/radix.hxx:195-470)
  * and extends it with values.  Bit order per inc/mgpu/common/cta_radix.hxx:
- * 6-64.  count < 2^32 - 12288 (32-bit positions).                          */
+ * 6-64.  count < 2^32 - 10240 (32-bit positions).                          */
 #define MGPU_DECL_RADIX(K, KT)                                                 \
   mgpu_status_t mgpu_radix_sort_keys_##K(void* aux, size_t* aux_size,          \
       mgpu_stream_t stream, KT* keys, int64_t count);                          \

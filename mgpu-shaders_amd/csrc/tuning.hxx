@@ -5,8 +5,9 @@
 template <typename key_t> struct merge_tuning { static constexpr int nt = 256, vt = 15; };
 template <typename key_t> struct mergesort_tuning { static constexpr int nt = 256, vt = 15; };
 // Radix: scatter block nt x vt (tile = nt*vt keys), count block cnt threads.
-// 512 x 24 (12288-key tiles) measured fastest for 2^28 u32 key-value on MI355X.
-template <typename key_t> struct radix_tuning { static constexpr int nt = 512, vt = 24, cnt = 1024; };
+// 512 x 20 (10240-key tiles) measured fastest for 2^28 u32 key-value on
+// MI355X (4.36 ms; 512 x 24: 4.47, 512 x 16: 4.54, 768 x 16: 4.68).
+template <typename key_t> struct radix_tuning { static constexpr int nt = 512, vt = 20, cnt = 512; };
 template <> struct radix_tuning<uint64_t> { static constexpr int nt = 512, vt = 12, cnt = 512; };
 // Scan: 256 x 80 (20480-element tiles).  Larger tiles mean fewer look-back
 // steps per byte; 2^29 int32 exclusive: 0.775 ms at 256 x 80 vs 1.19 ms at

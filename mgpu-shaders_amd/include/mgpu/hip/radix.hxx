@@ -120,6 +120,8 @@ struct radix_scatter_storage_t {
     uint64_t match[NW][radix_num_bins];    // per-wave lane masks of each digit (ranking)
   };
   uint32_t hist[NW][radix_num_bins];       // per-wave counts -> wave prefixes
+  uint32_t hist_sink[NW][warp_size];       // write target of lanes that update nothing
+  uint64_t match_sink[NW][warp_size];
   uint32_t local_off[radix_num_bins];      // tile-local exclusive digit offsets
   uint32_t global_off[radix_num_bins];     // output position of local slot 0 per digit
   uint32_t scan_tot[NW];
@@ -128,7 +130,16 @@ struct radix_scatter_storage_t {
 // Stable scatter of one tile by one digit.  Thread d < 256 passes `goff`,
 // the output position of the tile's first key with digit d (relative to
 // out_base); positions are checked against out_limit.
-template <int NT, int VT, typename key_t, bool has_values, typename xbits_t>
+//
+// `full` (tile_count == NT * VT) drops every clamp and guard.  Otherwise one
+// branch-free path serves partial tiles too: the lanes past the end load the
+// last key but take digit 255.  They are the
+// last keys of the tile in input order, so the stable ranking places them
+// after every real key of digit 255, i.e. exactly at tile positions
+// [tile_count, NT * VT), and their stores are simply predicated off.
+// (Divergent branches between an LDS read and its use would make the
+// compiler wait for every outstanding LDS operation.)
+template <int NT, int VT, typename key_t, bool has_values, bool full = false, typename xbits_t>
 __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
                                          key_t* keys_out, uint32_t* vals_out, int tile_count,
                                          int shift, xbits_t xb, uint32_t goff, int64_t out_limit,
@@ -137,28 +148,26 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   static_assert(NT >= radix_num_bins && NT % radix_num_bins == 0, "NT must cover the bins");
   static_assert(sizeof(key_t) * NT * VT >= sizeof(uint64_t) * NW * radix_num_bins,
                 "match table must fit in the key buffer");
+  static_assert(NT * VT <= 65536, "ranks are packed in 16 bits");
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int warp = tid / warp_size;
+  const uint32_t last = (uint32_t)tile_count - 1;
 
   // Each wave owns a contiguous chunk of 64*VT keys; slot i holds keys
   // [chunk + 64 i, chunk + 64 i + 64) in lane order, so wave-local rank order
   // (slot, lane) is the input order.
-  const int chunk = warp * warp_size * VT;
+  const uint32_t chunk = (uint32_t)(warp * warp_size * VT) + (uint32_t)lane;
   key_t k[VT];
-  uint32_t v[VT];
 #pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    int j = chunk + i * warp_size + lane;
-    if (j < tile_count) k[i] = keys_in[j];
-  }
-  if constexpr (has_values) {
-#pragma unroll
-    for (int i = 0; i < VT; ++i) {
-      int j = chunk + i * warp_size + lane;
-      v[i] = j < tile_count ? vals_in[j] : 0u;
-    }
-  }
+  for (int i = 0; i < VT; ++i)
+    k[i] = full ? (keys_in + chunk)[i * warp_size]  // constant offsets from one address
+                : keys_in[min_(chunk + (uint32_t)(i * warp_size), last)];
+  auto digit = [&](int i) -> uint32_t {
+    return full || chunk + (uint32_t)(i * warp_size) <= last
+               ? ((uint32_t)(xb(k[i]) >> shift) & 255u)
+               : 255u;
+  };
   // Wave-private counters and match table (LDS operations of one wave are
   // performed in order, so no block barrier is needed before use).
 #pragma unroll
@@ -167,34 +176,35 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
     s.match[warp][b] = 0;
   }
 
-  // Wave-local stable rank: the lanes sharing the first active lane's digit
-  // get their mask from one ballot; every other lane ORs its bit into the
-  // table entry of its digit and reads the entry back.  The lowest lane of
-  // each digit advances the wave's running count (and clears the entry).
+  // Wave-local stable rank: the lanes sharing the first lane's digit get
+  // their mask from one ballot; every other lane ORs its bit into the table
+  // entry of its digit and reads the entry back.  The lowest lane of each
+  // digit advances the wave's running count and clears the entry; the other
+  // lanes write the same values to private sink slots, so the writes need no
+  // branch.
   const uint64_t lane_bit = 1ull << lane;
   uint32_t r[VT];
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
-    const int j = chunk + i * warp_size + lane;
-    uint32_t rank = 0;
-    if (j < tile_count) {
-      const uint32_t d = (uint32_t)(xb(k[i]) >> shift) & 255u;
-      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-      const uint64_t same = __ballot(d == d0);
-      uint64_t m = same;
-      if (d != d0) {
-        atomicOr((unsigned long long*)&s.match[warp][d], (unsigned long long)lane_bit);
-        m = s.match[warp][d];
-      }
-      const uint32_t below = mask_count_below(m);
-      const uint32_t c = s.hist[warp][d];
-      rank = c + below;
-      if (below == 0) {
-        s.hist[warp][d] = c + (uint32_t)__popcll(m);
-        if (d != d0) s.match[warp][d] = 0;
-      }
+    const uint32_t d = digit(i);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t same = __ballot(d == d0);
+    const bool tab = d != d0;
+    // The counter read goes first so that it is in flight together with the
+    // table read (one LDS round trip per slot).
+    const uint32_t c = s.hist[warp][d];
+    uint64_t m = same;
+    if (tab) {
+      atomicOr((unsigned long long*)&s.match[warp][d], (unsigned long long)lane_bit);
+      m = s.match[warp][d];
     }
-    r[i] = rank;
+    const uint32_t below = mask_count_below(m);
+    r[i] = (d << 16) | (c + below);  // the digit rides along for the reorder
+    const bool lead = below == 0;
+    uint32_t* hp = lead ? &s.hist[warp][d] : &s.hist_sink[warp][lane];
+    *hp = c + (uint32_t)__popcll(m);
+    uint64_t* mp = lead && tab ? &s.match[warp][d] : &s.match_sink[warp][lane];
+    *mp = 0ull;
   }
   __syncthreads();
 
@@ -222,65 +232,81 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   }
   __syncthreads();
 
+  // Values are loaded only now: their latency overlaps the key reorder and
+  // stores, and they are not live during the ranking.
+  // The index base goes through an empty asm so that the compiler does not
+  // keep the keys' load offsets alive (in registers) to reuse them here.
+  uint32_t v[VT];
+  if constexpr (has_values) {
+    uint32_t vchunk = chunk;
+    asm volatile("" : "+v"(vchunk));
+#pragma unroll
+    for (int i = 0; i < VT; ++i)
+      v[i] = full ? (vals_in + vchunk)[i * warp_size]
+                  : vals_in[min_(vchunk + (uint32_t)(i * warp_size), last)];
+  }
+
   // Reorder the tile by digit in LDS.
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
-    const int jj = chunk + i * warp_size + lane;
-    if (jj < tile_count) {
-      const uint32_t d = (uint32_t)(xb(k[i]) >> shift) & 255u;
-      const uint32_t pos = s.local_off[d] + s.hist[warp][d] + r[i];
-      s.keys[pos] = k[i];
-      r[i] = pos;
-    }
+    const uint32_t d = r[i] >> 16;
+    const uint32_t pos = s.local_off[d] + s.hist[warp][d] + (r[i] & 0xFFFFu);
+    s.keys[pos] = k[i];
+    r[i] = pos;
   }
   __syncthreads();
 
   uint32_t dst[VT];
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
-    const int p = i * NT + tid;
-    dst[i] = 0xFFFFFFFFu;
-    if (p < tile_count) {
-      key_t key = s.keys[p];
-      const uint32_t d = (uint32_t)(xb(key) >> shift) & 255u;
-      dst[i] = s.global_off[d] + (uint32_t)p;
-      if (dst[i] < out_limit) keys_out[dst[i]] = key;
-    }
+    const uint32_t p = (uint32_t)(i * NT + tid);
+    const key_t key = s.keys[p];
+    const uint32_t d = (uint32_t)(xb(key) >> shift) & 255u;
+    dst[i] = s.global_off[d] + p;
+    if ((full || p <= last) && dst[i] < out_limit) keys_out[dst[i]] = key;
   }
   if constexpr (has_values) {
     __syncthreads();
     uint32_t* s_vals = (uint32_t*)s.keys;
 #pragma unroll
-    for (int i = 0; i < VT; ++i) {
-      const int jj = chunk + i * warp_size + lane;
-      if (jj < tile_count) s_vals[r[i]] = v[i];
-    }
+    for (int i = 0; i < VT; ++i) s_vals[r[i]] = v[i];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
-      const int p = i * NT + tid;
-      if (p < tile_count && dst[i] < out_limit) vals_out[dst[i]] = s_vals[p];
+      const uint32_t p = (uint32_t)(i * NT + tid);
+      if ((full || p <= last) && dst[i] < out_limit) vals_out[dst[i]] = s_vals[p];
     }
   }
 }
 
 // Scatter of one digit pass.  offsets[d * num_tiles + t] is the global
-// position of tile t's first key with digit d.
+// position of tile t's first key with digit d.  Blocks [0, num_full) take
+// the full tiles (dealt XCD-contiguously); a block num_full, if launched,
+// takes the trailing partial tile.
+// Two blocks per CU (the LDS allows two): at most 128 VGPRs.
 template <int NT, int VT, typename key_t, bool has_values>
-__global__ __launch_bounds__(NT) void kernel_radix_scatter(
+__global__ __launch_bounds__(NT, 2 * NT / 256) void kernel_radix_scatter(
     const key_t* keys_in, key_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-    int64_t count, int shift, const uint32_t* offsets, int64_t num_tiles) {
+    int64_t count, int shift, const uint32_t* offsets, int64_t num_tiles, int64_t num_full) {
   constexpr int NV = NT * VT;
   __shared__ radix_scatter_storage_t<NT, VT, key_t> s;
   const int tid = threadIdx.x;
-  const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
+  const bool full = blockIdx.x < num_full;
+  const int64_t tile = full ? xcd_tile(blockIdx.x, num_full) : num_full;
   const int64_t base = tile * NV;
   const int tile_count = (int)min_<int64_t>(NV, count - base);
   uint32_t goff = 0;
   if (tid < radix_num_bins) goff = offsets[tid * num_tiles + tile];
-  cta_radix_scatter<NT, VT, key_t, has_values>(keys_in + base, has_values ? vals_in + base : vals_in,
-                                               keys_out, vals_out, tile_count, shift,
-                                               radix_bits_t<key_t>(), goff, count, s);
+  const key_t* kin = keys_in + base;
+  const uint32_t* vin = has_values ? vals_in + base : vals_in;
+  if (full)
+    cta_radix_scatter<NT, VT, key_t, has_values, true>(kin, vin, keys_out, vals_out, tile_count,
+                                                       shift, radix_bits_t<key_t>(), goff,
+                                                       count, s);
+  else
+    cta_radix_scatter<NT, VT, key_t, has_values, false>(kin, vin, keys_out, vals_out,
+                                                        tile_count, shift, radix_bits_t<key_t>(),
+                                                        goff, count, s);
 }
 
 template <int NT, int VT, typename key_t, typename val_t>
@@ -338,10 +364,11 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
     mgpu_status_t st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)counts, counts,
                                      num_counts, MGPU_SCAN_EXCLUSIVE, 0u);
     if (st != MGPU_OK) return st;
+    const int64_t num_full = count / NV;
     e = launch_kernel("mgpu_radix_scatter", kernel_radix_scatter<NT, VT, key_t, has_values>,
                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, (const key_t*)k_in, k_out,
                       (const uint32_t*)v_in, v_out, count, shift, (const uint32_t*)counts,
-                      num_tiles);
+                      num_tiles, num_full);
     if (e != hipSuccess) return hip_status(e);
     std::swap(k_in, k_out);
     std::swap(v_in, v_out);

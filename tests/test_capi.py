@@ -75,9 +75,9 @@ def test_sizing_radix_and_scan():
     lib = mgpu_hip.load()
     n = 1 << 20
     st, sz = _size(lib.mgpu_radix_sort_kv_u32_u32, ctypes.c_void_p(1), ctypes.c_void_p(1), n)
-    tiles = -(-n // (512 * 24))
+    tiles = -(-n // (512 * 20))
     assert st == 0
-    scan_aux = r256(8 * (-(-tiles * 256 // 4096)))
+    scan_aux = r256(8 * (-(-tiles * 256 // 4096)))  # counts scan: 4096-element tiles
     assert sz == r256(4 * n) * 2 + r256(4 * tiles * 256) + scan_aux
     st, sz = _size(lib.mgpu_scan_exclusive_i32, ctypes.c_void_p(1), n, 0)
     assert st == 0 and sz == r256(8 * -(-n // (256 * 80)))  # 8 B per 20480-element tile

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 KTS = ("u32", "i32", "u64", "f32")
 MS_NV = 256 * 15      # mergesort tile of the C ABI instantiation
-RADIX_NV = 512 * 24   # radix tile (u32/i32/f32)
+RADIX_NV = 512 * 20   # radix tile (u32/i32/f32)
 SIZES = [1, 2, 7, 64, 100, MS_NV - 1, MS_NV, MS_NV + 1, 2 * MS_NV + 3, 5 * MS_NV - 11,
          RADIX_NV + 1, 100003, (1 << 20) + 17]
 
