@@ -2,10 +2,10 @@
 //
 // merge_path follows inc/mgpu/common/cta_merge.hxx:10-31 with bounds_lower
 // (ties go to A): it returns the number of A elements among the first `diag`
-// outputs of the stable merge.  The partition search that the reference runs
-// as one binary search per thread in global memory (vk/partition.hxx:9-20) is
-// done here by a whole wave: 64 lanes probe 64 evenly spaced points, so a
-// 2^27-element range resolves in 5 dependent rounds instead of 27.
+// outputs of the stable merge.  The partition search is one binary search
+// per thread in global memory, as in vk/partition.hxx:9-20 (a wave-wide
+// 64-probe search was measured: fewer dependent rounds, but four times the
+// memory traffic, and slower).
 //
 // A tile merge (kernel_merge.hxx:9-58 and cta_merge.hxx:136-162) stages the A
 // and B slices of one output tile in LDS with coalesced loads, runs one merge
@@ -109,20 +109,32 @@ __device__ inline void cta_merge_tile(
   a_cnt = min_(a_cnt, total);
   b_cnt = total - a_cnt;
 
-  // Load both streams into LDS (coalesced strided loads).
-#pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    int j = i * NT + tid;
-    if (j < a_cnt) s.keys[j] = a_keys[a_begin + j];
-    else if (j < total) s.keys[j] = b_keys[b_begin + (j - a_cnt)];
-  }
-  if constexpr (has_values) {
+  if (total <= 0) return;
+
+  // Load both streams into LDS (coalesced strided loads).  Each load reads a
+  // clamped index from A or B chosen by select, with no branch, so that all
+  // loads are in flight together (a guarded load costs one wait each).
+  {
+    key_t x[VT];
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
-      int j = i * NT + tid;
-      if (j < a_cnt) s.vals[j] = a_vals[a_begin + j];
-      else if (j < total) s.vals[j] = b_vals[b_begin + (j - a_cnt)];
+      const int j = min_(i * NT + tid, total - 1);
+      const bool in_a = j < a_cnt;
+      x[i] = (in_a ? a_keys : b_keys)[in_a ? a_begin + j : b_begin + (j - a_cnt)];
     }
+#pragma unroll
+    for (int i = 0; i < VT; ++i) s.keys[i * NT + tid] = x[i];
+  }
+  if constexpr (has_values) {
+    val_t x[VT];
+#pragma unroll
+    for (int i = 0; i < VT; ++i) {
+      const int j = min_(i * NT + tid, total - 1);
+      const bool in_a = j < a_cnt;
+      x[i] = (in_a ? a_vals : b_vals)[in_a ? a_begin + j : b_begin + (j - a_cnt)];
+    }
+#pragma unroll
+    for (int i = 0; i < VT; ++i) s.vals[i * NT + tid] = x[i];
   }
   __syncthreads();
 
@@ -148,14 +160,16 @@ __device__ inline void cta_merge_tile(
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
-    int j = i * NT + tid;
-    if (j < total) c_keys[c_begin + j] = s.keys[j];
+    const int j = i * NT + tid;
+    const key_t x = s.keys[j];
+    if (j < total) c_keys[c_begin + j] = x;
   }
   if constexpr (has_values) {
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
-      int j = i * NT + tid;
-      if (j < total) c_vals[c_begin + j] = s.vals[j];
+      const int j = i * NT + tid;
+      const val_t x = s.vals[j];
+      if (j < total) c_vals[c_begin + j] = x;
     }
   }
 }

@@ -122,17 +122,23 @@ __device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in,
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   constexpr int NV = NT * VT;
   const int tid = threadIdx.x;
+  if (tile_count <= 0) return;
+  // Unguarded loads of a clamped index (slots past the end get a copy of the
+  // last key; the out-of-range mask keeps them out of the sort), so that all
+  // loads are in flight together instead of one wait per guarded load.
+  {
+    key_t x[VT];
 #pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    int j = i * NT + tid;
-    if (j < tile_count) s.keys[j] = keys_in[j];
+    for (int i = 0; i < VT; ++i) x[i] = keys_in[min_(i * NT + tid, tile_count - 1)];
+#pragma unroll
+    for (int i = 0; i < VT; ++i) s.keys[i * NT + tid] = x[i];
   }
   if constexpr (has_values && !sort_indices) {
+    val_t x[VT];
 #pragma unroll
-    for (int i = 0; i < VT; ++i) {
-      int j = i * NT + tid;
-      if (j < tile_count) s.vals[j] = vals_in[j];
-    }
+    for (int i = 0; i < VT; ++i) x[i] = vals_in[min_(i * NT + tid, tile_count - 1)];
+#pragma unroll
+    for (int i = 0; i < VT; ++i) s.vals[i * NT + tid] = x[i];
   }
   __syncthreads();
   key_t k[VT];
@@ -170,14 +176,16 @@ __device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in,
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
-    int j = i * NT + tid;
-    if (j < tile_count) keys_out[j] = s.keys[j];
+    const int j = i * NT + tid;
+    const key_t x = s.keys[j];
+    if (j < tile_count) keys_out[j] = x;
   }
   if constexpr (has_values) {
 #pragma unroll
     for (int i = 0; i < VT; ++i) {
-      int j = i * NT + tid;
-      if (j < tile_count) vals_out[j] = s.vals[j];
+      const int j = i * NT + tid;
+      const val_t x = s.vals[j];
+      if (j < tile_count) vals_out[j] = x;
     }
   }
 }
