@@ -78,6 +78,16 @@ struct mgpu_server_st {
 
 namespace {
 
+// Stable key-value merge of two ascending arrays of n keys, with the tuned
+// tile of the key type (aux == nullptr: sizing).
+template <typename K>
+mgpu_status_t serve_merge(void* aux, size_t& cap, hipStream_t s, const void* a, const void* b,
+                          const uint32_t* va, int64_t n, void* c, uint32_t* vc) {
+  return merge<merge_tuning<K>::nt, merge_tuning<K>::vt>(
+      aux, cap, s, (const K*)a, va, n, (const K*)b, va ? va + n : nullptr, n, (K*)c, vc,
+      less_t<K>());
+}
+
 // Device bytes a request needs (buffers + aux), from the sizing calls.
 size_t request_bytes(const mgpu_request_t& r) {
   const int64_t n = r.n;
@@ -91,9 +101,10 @@ size_t request_bytes(const mgpu_request_t& r) {
         radix_sort<radix_tuning<uint32_t>::nt, radix_tuning<uint32_t>::vt, radix_tuning<uint32_t>::cnt>(nullptr, aux, nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
       return 2 * aux_round(n * kb) + aux_round(n * 4) + aux_round(aux);
     case MGPU_REQ_MERGE:
-      merge<256, 15>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n,
-                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, (uint32_t*)nullptr,
-                     (uint32_t*)nullptr, less_t<uint32_t>());
+      if (r.key_type == MGPU_KEY_U64)
+        serve_merge<uint64_t>(nullptr, aux, nullptr, nullptr, nullptr, nullptr, n, nullptr, nullptr);
+      else
+        serve_merge<uint32_t>(nullptr, aux, nullptr, nullptr, nullptr, nullptr, n, nullptr, nullptr);
       return 2 * aux_round(n * kb) + aux_round(2 * n * 4) + aux_round(2 * n * kb) +
              aux_round(2 * n * 4) + aux_round(aux);
     case MGPU_REQ_SCAN:
@@ -161,9 +172,10 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
     case MGPU_REQ_MERGE: {
       const int kind = q.key_type == MGPU_KEY_U64 ? gen_sorted_u64
                        : q.key_type == MGPU_KEY_F32 ? gen_sorted_f32 : gen_sorted_u32;
-      merge<256, 15>(nullptr, aux, s, (const uint32_t*)nullptr, (const uint32_t*)nullptr, n,
-                     (const uint32_t*)nullptr, (const uint32_t*)nullptr, n, (uint32_t*)nullptr,
-                     (uint32_t*)nullptr, less_t<uint32_t>());
+      if (q.key_type == MGPU_KEY_U64)
+        serve_merge<uint64_t>(nullptr, aux, s, nullptr, nullptr, nullptr, n, nullptr, nullptr);
+      else
+        serve_merge<uint32_t>(nullptr, aux, s, nullptr, nullptr, nullptr, n, nullptr, nullptr);
       if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->in_b, n * kb, s)) ||
           !ok(alloc(r, &r->vals_a, 2 * n * 4, s)) || !ok(alloc(r, &r->out, 2 * n * kb, s)) ||
           !ok(alloc(r, &r->vals, 2 * n * 4, s)) || !ok(alloc(r, &r->aux, aux, s)))
@@ -174,24 +186,16 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
       if (e != hipSuccess) break;
       size_t cap = aux_round(aux ? aux : 1);
       const uint32_t* va = (const uint32_t*)r->vals_a;
-      const uint32_t* vb = va + n;
       uint32_t* vc = (uint32_t*)r->vals;
       switch (q.key_type) {
         case MGPU_KEY_U64:
-          return merge<256, 15>(r->aux, cap, s, (const uint64_t*)r->in_a, va, n,
-                                (const uint64_t*)r->in_b, vb, n, (uint64_t*)r->out, vc,
-                                less_t<uint64_t>());
+          return serve_merge<uint64_t>(r->aux, cap, s, r->in_a, r->in_b, va, n, r->out, vc);
         case MGPU_KEY_F32:
-          return merge<256, 15>(r->aux, cap, s, (const float*)r->in_a, va, n,
-                                (const float*)r->in_b, vb, n, (float*)r->out, vc, less_t<float>());
+          return serve_merge<float>(r->aux, cap, s, r->in_a, r->in_b, va, n, r->out, vc);
         case MGPU_KEY_I32:
-          return merge<256, 15>(r->aux, cap, s, (const int32_t*)r->in_a, va, n,
-                                (const int32_t*)r->in_b, vb, n, (int32_t*)r->out, vc,
-                                less_t<int32_t>());
+          return serve_merge<int32_t>(r->aux, cap, s, r->in_a, r->in_b, va, n, r->out, vc);
         default:
-          return merge<256, 15>(r->aux, cap, s, (const uint32_t*)r->in_a, va, n,
-                                (const uint32_t*)r->in_b, vb, n, (uint32_t*)r->out, vc,
-                                less_t<uint32_t>());
+          return serve_merge<uint32_t>(r->aux, cap, s, r->in_a, r->in_b, va, n, r->out, vc);
       }
     }
     case MGPU_REQ_SCAN: {

@@ -50,7 +50,7 @@ def r256(x):
 def test_sizing_mergesort_matches_formula():
     lib = mgpu_hip.load()
     n = 1 << 20
-    nv = 256 * 15
+    nv = 512 * 19  # mergesort tile of 32-bit keys
     tiles = -(-n // nv)
     st, sz = _size(lib.mgpu_mergesort_kv_u32_u32, ctypes.c_void_p(1), ctypes.c_void_p(1), n, 0)
     assert st == 0
@@ -68,7 +68,7 @@ def test_sizing_accumulates():
     st = lib.mgpu_merge_keys_u32(None, ctypes.byref(size), None, ctypes.c_void_p(1), 5000,
                                  ctypes.c_void_p(1), 5000, ctypes.c_void_p(1), 0)
     assert st == 0
-    assert size.value == 1000 + r256(8 * (-(-10000 // (256 * 15)) + 1))
+    assert size.value == 1000 + r256(8 * (-(-10000 // (512 * 23)) + 1))  # merge tile
 
 
 def test_sizing_radix_and_scan():

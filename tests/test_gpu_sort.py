@@ -10,10 +10,12 @@ from gpu_util import banner, dev_equal, gen_dev, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 KTS = ("u32", "i32", "u64", "f32")
-MS_NV = 256 * 15      # mergesort tile of the C ABI instantiation
+MS_NV = 512 * 19      # mergesort tile of 32-bit keys in the C ABI instantiation
+MS_NV64 = 256 * 15    # mergesort and merge tile of 64-bit keys
+MERGE_NV = 512 * 23   # merge tile of 32-bit keys
 RADIX_NV = 512 * 20   # radix tile (u32/i32/f32)
-SIZES = [1, 2, 7, 64, 100, MS_NV - 1, MS_NV, MS_NV + 1, 2 * MS_NV + 3, 5 * MS_NV - 11,
-         RADIX_NV + 1, 100003, (1 << 20) + 17]
+SIZES = [1, 2, 7, 64, 100, MS_NV64 - 1, MS_NV64, MS_NV64 + 1, MS_NV - 1, MS_NV, MS_NV + 1,
+         2 * MS_NV + 3, 5 * MS_NV - 11, RADIX_NV + 1, 100003, (1 << 20) + 17]
 
 
 def make_keys(kt, n, seed, pattern="random"):
@@ -103,7 +105,8 @@ def test_mergesort_vk_sort_known_answer():
 
 # ---------------------------------------------------------------------- merge
 
-MERGE_SIZES = [(0, 1), (1, 0), (1, 1), (5, 3000), (3000, 5), (MS_NV, MS_NV), (10007, 20011),
+MERGE_SIZES = [(0, 1), (1, 0), (1, 1), (5, 3000), (3000, 5), (MS_NV64, MS_NV64),
+               (MERGE_NV // 2, MERGE_NV // 2), (MERGE_NV, 1), (10007, 20011),
                (1 << 19, (1 << 19) + 3)]
 
 
