@@ -378,6 +378,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-segsort", action="store_true")
     ap.add_argument("--cpu-log2", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -449,11 +451,13 @@ def main():
     del keys_src, vals_src, keys, vals, aux
     torch.cuda.empty_cache()
     if not args.no_secondary:
-        result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
-        torch.cuda.empty_cache()
+        if not args.no_segsort:
+            result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
+            torch.cuda.empty_cache()
         result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
         torch.cuda.empty_cache()
-        result["serving"] = bench_serving(torch, mgpu_hip)
+        if not args.no_serving:
+            result["serving"] = bench_serving(torch, mgpu_hip)
     if not args.no_cpu:
         log("cpu baseline 2^%d" % args.cpu_log2)
         result["cpu_baseline"] = cpu_baseline(args.cpu_log2)
