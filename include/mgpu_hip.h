@@ -79,6 +79,8 @@ MGPU_DECL_MERGESORT(u32, uint32_t)
 MGPU_DECL_MERGESORT(i32, int32_t)
 MGPU_DECL_MERGESORT(u64, uint64_t)
 MGPU_DECL_MERGESORT(f32, float)
+MGPU_DECL_MERGESORT(i64, int64_t)
+MGPU_DECL_MERGESORT(f64, double)
 
 /* ------------------------------------------------------------------------ */
 /* Two-way stable merge (ties take A first), out of place into c.
@@ -97,6 +99,8 @@ MGPU_DECL_MERGE(u32, uint32_t)
 MGPU_DECL_MERGE(i32, int32_t)
 MGPU_DECL_MERGE(u64, uint64_t)
 MGPU_DECL_MERGE(f32, float)
+MGPU_DECL_MERGE(i64, int64_t)
+MGPU_DECL_MERGE(f64, double)
 
 /* ------------------------------------------------------------------------ */
 /* LSD radix sort, 8-bit digits, ascending in radix-bit order (unsigned:
@@ -113,6 +117,8 @@ MGPU_DECL_RADIX(u32, uint32_t)
 MGPU_DECL_RADIX(i32, int32_t)
 MGPU_DECL_RADIX(u64, uint64_t)
 MGPU_DECL_RADIX(f32, float)
+MGPU_DECL_RADIX(i64, int64_t)
+MGPU_DECL_RADIX(f64, double)
 
 /* ------------------------------------------------------------------------ */
 /* Scan with plus.
@@ -194,6 +200,8 @@ MGPU_DECL_SEGSORT(u32, uint32_t)
 MGPU_DECL_SEGSORT(i32, int32_t)
 MGPU_DECL_SEGSORT(u64, uint64_t)
 MGPU_DECL_SEGSORT(f32, float)
+MGPU_DECL_SEGSORT(i64, int64_t)
+MGPU_DECL_SEGSORT(f64, double)
 
 /* ------------------------------------------------------------------------ */
 /* Graph capture of any sequence of library calls on one stream (no

@@ -41,3 +41,5 @@ MGPU_DEF_MERGE(u32, uint32_t)
 MGPU_DEF_MERGE(i32, int32_t)
 MGPU_DEF_MERGE(u64, uint64_t)
 MGPU_DEF_MERGE(f32, float)
+MGPU_DEF_MERGE(i64, int64_t)
+MGPU_DEF_MERGE(f64, double)

@@ -52,3 +52,5 @@ MGPU_DEF_MERGESORT(u32, uint32_t)
 MGPU_DEF_MERGESORT(i32, int32_t)
 MGPU_DEF_MERGESORT(u64, uint64_t)
 MGPU_DEF_MERGESORT(f32, float)
+MGPU_DEF_MERGESORT(i64, int64_t)
+MGPU_DEF_MERGESORT(f64, double)

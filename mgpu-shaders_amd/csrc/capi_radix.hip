@@ -24,3 +24,5 @@ MGPU_DEF_RADIX(u32, uint32_t)
 MGPU_DEF_RADIX(i32, int32_t)
 MGPU_DEF_RADIX(u64, uint64_t)
 MGPU_DEF_RADIX(f32, float)
+MGPU_DEF_RADIX(i64, int64_t)
+MGPU_DEF_RADIX(f64, double)

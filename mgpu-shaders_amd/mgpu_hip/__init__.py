@@ -27,12 +27,12 @@ LESS, GREATER = 0, 1
 EXCLUSIVE, INCLUSIVE = 0, 1
 GEN_KINDS = {"u32": 0, "lowentropy": 1, "i32_0_16": 2, "f32_01": 3, "u64": 4, "f32_pm1": 5,
              "iota": 6, "sorted_u32": 7, "sorted_u64": 8, "sorted_f32": 9}
-KEY_TYPES = ("u32", "i32", "u64", "f32")
+KEY_TYPES = ("u32", "i32", "u64", "f32", "i64", "f64")
 _CTYPE = {"u32": ctypes.c_uint32, "i32": ctypes.c_int32, "u64": ctypes.c_uint64,
-          "f32": ctypes.c_float}
+          "f32": ctypes.c_float, "i64": ctypes.c_int64, "f64": ctypes.c_double}
 _DTYPE_KEY = {torch.int32: "i32", torch.float32: "f32", torch.uint32: "u32",
-              torch.uint64: "u64"}
-_KEY_BYTES = {"u32": 4, "i32": 4, "u64": 8, "f32": 4}
+              torch.uint64: "u64", torch.int64: "i64", torch.float64: "f64"}
+_KEY_BYTES = {"u32": 4, "i32": 4, "u64": 8, "f32": 4, "i64": 8, "f64": 8}
 
 
 class MgpuError(RuntimeError):

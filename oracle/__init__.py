@@ -29,8 +29,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
-KT = {"u32": 0, "i32": 1, "u64": 2, "f32": 3}
-NP = {"u32": np.uint32, "i32": np.int32, "u64": np.uint64, "f32": np.float32}
+KT = {"u32": 0, "i32": 1, "u64": 2, "f32": 3, "i64": 4, "f64": 5}
+NP = {"u32": np.uint32, "i32": np.int32, "u64": np.uint64, "f32": np.float32, "i64": np.int64,
+      "f64": np.float64}
+WIDE = ("u64", "i64", "f64")
 GEN = {"u32": 0, "lowentropy": 1, "i32_0_16": 2, "f32_01": 3, "u64": 4, "f32_pm1": 5, "iota": 6}
 
 _lib = None
@@ -121,6 +123,23 @@ def stable_sort(keys, vals=None, kt="u32", greater=False):
     v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
     lib().oracle_stable_sort(KT[kt], _p(k), _p(v), len(k), int(greater))
     return (k, v) if vals is not None else k
+
+
+def f64_from_u64(x):
+    """Finite doubles in [-1, 1) from 64-bit words (53 high bits), for key tests."""
+    return (np.asarray(x, dtype=np.uint64) >> np.uint64(11)).astype(np.float64) * 2.0 ** -52 - 1.0
+
+
+def wide_keys(kt, x):
+    """Keys of type kt from generator words x (u32 words for 4-byte types,
+    u64 words for 8-byte types)."""
+    if kt == "i32":
+        return x.view(np.int32)
+    if kt == "i64":
+        return x.view(np.int64)
+    if kt == "f64":
+        return f64_from_u64(x)
+    return x
 
 
 def segmented_stable_sort(keys, segments, vals=None, kt="u32", greater=False):
@@ -270,11 +289,11 @@ def radix_schedule(keys, vals=None, nv=4096, kt="u32", want_first_pass=False):
 
 def radix_bits(keys, kt="f32", inverse=False):
     if inverse:
-        src = np.ascontiguousarray(keys, dtype=np.uint64 if kt == "u64" else np.uint32)
+        src = np.ascontiguousarray(keys, dtype=np.uint64 if kt in WIDE else np.uint32)
         out = np.empty(len(src), dtype=NP[kt])
     else:
         src = np.ascontiguousarray(keys, dtype=NP[kt])
-        out = np.empty(len(src), dtype=np.uint64 if kt == "u64" else np.uint32)
+        out = np.empty(len(src), dtype=np.uint64 if kt in WIDE else np.uint32)
     lib().oracle_radix_bits(KT[kt], _p(src), _p(out), len(src), int(inverse))
     return out
 

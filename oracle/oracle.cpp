@@ -49,7 +49,7 @@ namespace {
 //   4: scan schedule drops the carry-in of the downsweep
 int g_plant = 0;
 
-enum { KT_U32 = 0, KT_I32 = 1, KT_U64 = 2, KT_F32 = 3 };
+enum { KT_U32 = 0, KT_I32 = 1, KT_U64 = 2, KT_F32 = 3, KT_I64 = 4, KT_F64 = 5 };
 
 // ---------------------------------------------------------------- generators
 uint64_t mix64(uint64_t z) {
@@ -67,6 +67,8 @@ template <> struct ubits<uint32_t> { typedef uint32_t u; typedef int32_t s; };
 template <> struct ubits<int32_t> { typedef uint32_t u; typedef int32_t s; };
 template <> struct ubits<uint64_t> { typedef uint64_t u; typedef int64_t s; };
 template <> struct ubits<float> { typedef uint32_t u; typedef int32_t s; };
+template <> struct ubits<int64_t> { typedef uint64_t u; typedef int64_t s; };
+template <> struct ubits<double> { typedef uint64_t u; typedef int64_t s; };
 
 template <typename K>
 typename ubits<K>::u to_bits(K x) {
@@ -76,8 +78,8 @@ typename ubits<K>::u to_bits(K x) {
   U y;
   std::memcpy(&y, &x, sizeof(y));
   if (std::is_same<K, uint32_t>::value || std::is_same<K, uint64_t>::value) return y;
-  if (std::is_same<K, int32_t>::value) return y ^ msb;
-  // float: flip everything when negative, else only the sign bit.
+  if (std::is_same<K, int32_t>::value || std::is_same<K, int64_t>::value) return y ^ msb;
+  // float / double: flip everything when negative, else only the sign bit.
   U mask = (U)((S)y >> (8 * sizeof(K) - 1)) | msb;
   return y ^ mask;
 }
@@ -86,8 +88,8 @@ K from_bits(typename ubits<K>::u y) {
   typedef typename ubits<K>::u U;
   typedef typename ubits<K>::s S;
   const U msb = (U)1 << (8 * sizeof(K) - 1);
-  if (std::is_same<K, int32_t>::value) y ^= msb;
-  if (std::is_same<K, float>::value) {
+  if (std::is_same<K, int32_t>::value || std::is_same<K, int64_t>::value) y ^= msb;
+  if (std::is_same<K, float>::value || std::is_same<K, double>::value) {
     y ^= msb;
     U mask = (U)((S)y >> (8 * sizeof(K) - 1)) & (msb - 1);
     y ^= mask;
@@ -400,6 +402,8 @@ void dispatch_key(int kt, F f) {
     case KT_I32: f((int32_t*)nullptr); break;
     case KT_U64: f((uint64_t*)nullptr); break;
     case KT_F32: f((float*)nullptr); break;
+    case KT_I64: f((int64_t*)nullptr); break;
+    case KT_F64: f((double*)nullptr); break;
   }
 }
 

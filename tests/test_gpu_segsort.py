@@ -12,14 +12,13 @@ from mgpu_hip import workloads
 
 pytestmark = pytest.mark.gpu
 
-KTS = ("u32", "i32", "u64", "f32")
+KTS = ("u32", "i32", "u64", "f32", "i64", "f64")
 
 
 def make_keys(kt, n, seed, ties=True):
-    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1"}[kt]
-    x = oracle.generate(n, kind, seed)
-    if kt == "i32":
-        x = x.view(np.int32)
+    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1", "i64": "u64",
+            "f64": "u64"}[kt]
+    x = oracle.wide_keys(kt, oracle.generate(n, kind, seed))
     if ties and n:
         x = x[np.arange(n) % 211]
     return x.copy()

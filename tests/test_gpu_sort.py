@@ -9,7 +9,7 @@ from gpu_util import banner, dev_equal, gen_dev, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-KTS = ("u32", "i32", "u64", "f32")
+KTS = ("u32", "i32", "u64", "f32", "i64", "f64")
 MS_NV = 512 * 19      # mergesort tile of 32-bit keys in the C ABI instantiation
 MS_NV64 = 256 * 15    # mergesort and merge tile of 64-bit keys
 MERGE_NV = 512 * 23   # merge tile of 32-bit keys
@@ -19,12 +19,11 @@ SIZES = [1, 2, 7, 64, 100, MS_NV64 - 1, MS_NV64, MS_NV64 + 1, MS_NV - 1, MS_NV, 
 
 
 def make_keys(kt, n, seed, pattern="random"):
-    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1"}[kt]
-    if pattern == "lowentropy" and kt != "u64" and kt != "f32":
+    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1", "i64": "u64",
+            "f64": "u64"}[kt]
+    if pattern == "lowentropy" and kt in ("u32", "i32"):
         kind = "lowentropy"
-    x = oracle.generate(n, kind, seed)
-    if kt == "i32":
-        x = x.view(np.int32)
+    x = oracle.wide_keys(kt, oracle.generate(n, kind, seed))
     if pattern == "ties":
         x = x[np.arange(n) % 97] if n else x
     elif pattern == "equal":
@@ -192,6 +191,28 @@ def test_radix_special_floats():
     mgpu_hip.radix_sort(dk, dv)
     ek, ev = oracle.stable_sort_radix(keys, vals, "f32")
     np.testing.assert_array_equal(to_host(dk).view(np.uint32), ek.view(np.uint32))
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
+def test_radix_special_doubles_and_int64_extremes():
+    banner("radix special doubles, int64 extremes")
+    base = np.array([np.nan, np.inf, 1.0, 0.0, -0.0, -1.0, -np.inf, 5e-324, -5e-324, 1.7e308,
+                     -1.7e308], dtype=np.float64)
+    neg_nan = np.array([0xFFF8000000000000], dtype=np.uint64).view(np.float64)
+    keys = np.tile(np.concatenate([base, neg_nan]), 2000)
+    vals = np.arange(len(keys), dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.radix_sort(dk, dv)
+    ek, ev = oracle.stable_sort_radix(keys, vals, "f64")
+    np.testing.assert_array_equal(to_host(dk).view(np.uint64), ek.view(np.uint64))
+    np.testing.assert_array_equal(to_host(dv), ev)
+    ints = np.tile(np.array([-2**63, -2**32, -1, 0, 1, 2**32, 2**63 - 1], dtype=np.int64), 5000)
+    ints = ints[oracle.generate(len(ints), "u32", 3).argsort(kind="stable")]
+    vals = np.arange(len(ints), dtype=np.uint32)
+    dk, dv = dev_keys(ints), to_dev(vals)
+    mgpu_hip.radix_sort(dk, dv)
+    ek, ev = oracle.stable_sort_radix(ints, vals, "i64")
+    np.testing.assert_array_equal(to_host(dk), ek)
     np.testing.assert_array_equal(to_host(dv), ev)
 
 

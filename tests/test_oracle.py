@@ -11,7 +11,7 @@ import pytest
 import oracle
 from mgpu_hip import workloads
 
-KTS = ("u32", "i32", "u64", "f32")
+KTS = ("u32", "i32", "u64", "f32", "i64", "f64")
 
 
 def np_splitmix_word(seed, stream, i):
@@ -26,18 +26,18 @@ def np_splitmix_word(seed, stream, i):
 
 
 def keys_for(kt, n, seed, low_entropy=False):
-    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1"}[kt]
+    kind = {"u32": "u32", "i32": "u32", "u64": "u64", "f32": "f32_pm1", "i64": "u64",
+            "f64": "u64"}[kt]
     if low_entropy and kt in ("u32", "i32"):
         kind = "lowentropy"
-    x = oracle.generate(n, kind, seed)
-    return x.view(oracle.NP[kt]) if kt == "i32" else x
+    return oracle.wide_keys(kt, oracle.generate(n, kind, seed))
 
 
 def np_stable_order(keys, kt, greater=False):
     if not greater:
         return np.argsort(keys, kind="stable")
-    if kt in ("u32", "u64"):
-        return np.argsort(~keys, kind="stable")
+    if kt in ("u32", "u64", "i32", "i64"):
+        return np.argsort(~np_radix_bits(keys, kt), kind="stable")
     return np.argsort(-(keys.astype(np.float64)), kind="stable")
 
 
@@ -46,6 +46,12 @@ def np_radix_bits(keys, kt):
         return keys
     if kt == "i32":
         return keys.view(np.uint32) ^ np.uint32(0x80000000)
+    if kt == "i64":
+        return keys.view(np.uint64) ^ np.uint64(1 << 63)
+    if kt == "f64":
+        b = keys.view(np.uint64)
+        mask = np.where(b >> np.uint64(63), np.uint64(0xFFFFFFFFFFFFFFFF), np.uint64(1 << 63))
+        return b ^ mask
     b = keys.view(np.uint32)
     mask = np.where(b >> np.uint32(31), np.uint32(0xFFFFFFFF), np.uint32(0x80000000))
     return b ^ mask
@@ -154,6 +160,30 @@ def test_radix_bits_signed_and_u64():
     assert np.all(b[:-1] < b[1:])
     u = np.array([0, 1, 2**63, 2**64 - 1], dtype=np.uint64)
     np.testing.assert_array_equal(oracle.radix_bits(u, "u64"), u)
+    # 64-bit signed keys: the sign flip is a 64-bit shift (the reference's
+    # int-literal shift, cta_radix.hxx:18,44, would be undefined here).
+    j = np.array([-2**63, -2**32, -1, 0, 1, 2**32, 2**63 - 1], dtype=np.int64)
+    bj = oracle.radix_bits(j, "i64")
+    assert np.all(bj[:-1] < bj[1:])
+    np.testing.assert_array_equal(bj, np_radix_bits(j, "i64"))
+    np.testing.assert_array_equal(oracle.radix_bits(bj, "i64", inverse=True), j)
+
+
+def test_radix_bits_special_doubles():
+    x = np.array([np.nan, np.inf, 1.0, 0.0, -0.0, -1.0, -np.inf, 5e-324, -5e-324, 1.7e308,
+                  -1.7e308], dtype=np.float64)
+    neg_nan = np.array([0xFFF8000000000000], dtype=np.uint64).view(np.float64)
+    x = np.concatenate([x, neg_nan])
+    bits = oracle.radix_bits(x, "f64")
+    np.testing.assert_array_equal(bits, np_radix_bits(x, "f64"))
+    back = oracle.radix_bits(bits, "f64", inverse=True)
+    np.testing.assert_array_equal(back.view(np.uint64), x.view(np.uint64))
+    order = x[np.argsort(bits, kind="stable")]
+    assert np.isnan(order[0]) and np.signbit(order[0]) and np.isnan(order[-1])
+    finite = order[1:-1]
+    assert np.all(finite[:-1] <= finite[1:])
+    iz = [i for i, v in enumerate(order) if v == 0]
+    assert np.signbit(order[iz[0]]) and not np.signbit(order[iz[1]])
 
 
 @pytest.mark.parametrize("kt", KTS)
