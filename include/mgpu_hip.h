@@ -154,6 +154,7 @@ MGPU_DECL_SCAN(u64, uint64_t)
  *   exclusive: out[i] = init + in[h] + ... + in[i-1]   (h = its segment start)
  *   inclusive: out[i] = init + in[h] + ... + in[i]
  * Single pass: a tile holding a segment start needs no carry beyond it.
+ * f32: float32 sums only inside aligned 256-element blocks, float64 above.
  * input == output is allowed.  count < 2^31.                               */
 #define MGPU_DECL_SEGSCAN(T, TT)                                               \
   mgpu_status_t mgpu_segmented_scan_##T(void* aux, size_t* aux_size,           \

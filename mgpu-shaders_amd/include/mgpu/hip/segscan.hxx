@@ -11,7 +11,12 @@
 //
 // Scan of (head flag, value) pairs under (fa, va) . (fb, vb) = (fa | fb,
 // fb ? vb : va + vb), in the hierarchical order of the plain scan (thread-
-// serial over 4 elements, a DPP wave scan, the wave totals, the carry).
+// serial over 4 elements, a DPP wave scan, the wave totals, the carry); as
+// there, every combination above an aligned 256-element wave block is taken
+// in the carry type (float64 for float32 data), so float32 results have at
+// most 10 float32 additions on their path and do not depend on the tile
+// shape.  Tiles are 256 x 32 by default (8192 elements): fewer tiles, fewer
+// look-backs per byte.
 // Tile heads come from a small search kernel, tile_first[t] = number of
 // segments starting before t * NV, so a tile's heads are
 // segments[tile_first[t] .. tile_first[t + 1]) and are marked in an LDS
@@ -171,13 +176,29 @@ __device__ inline seg_pair_t<acc_t> seg_combine(seg_pair_t<acc_t> a, seg_pair_t<
   return {a.f | b.f, b.f ? b.v : (acc_t)(a.v + b.v)};
 }
 
-// Result of one element from its in-tile prefix pair and the tile carry.
-__device__ inline uint32_t segscan_finish(uint32_t carry, uint32_t init, uint32_t f,
-                                          uint32_t local) {
-  return (f ? init : carry) + local;
+// (flag, value) pair in the carry type, for sums above a wave row.
+template <typename carry_t>
+struct seg_cpair_t {
+  uint32_t f;
+  carry_t v;
+};
+template <typename carry_t>
+__device__ inline seg_cpair_t<carry_t> seg_combine_c(seg_cpair_t<carry_t> a, uint32_t bf,
+                                                     carry_t bv) {
+  return {a.f | bf, bf ? bv : (carry_t)(a.v + bv)};
 }
-__device__ inline float segscan_finish(double carry, float init, uint32_t f, float local) {
-  return (float)((f ? (double)init : carry) + (double)local);
+
+// Result of one element: `pre` is the pair of everything in the tile before
+// its wave row segment (carry type), `e` its prefix inside the segment.
+__device__ inline uint32_t segscan_finish(uint32_t carry, uint32_t init,
+                                          seg_cpair_t<uint32_t> pre, seg_pair_t<uint32_t> e) {
+  if (e.f) return init + e.v;
+  return (pre.f ? init : carry) + pre.v + e.v;
+}
+__device__ inline float segscan_finish(double carry, float init, seg_cpair_t<double> pre,
+                                       seg_pair_t<float> e) {
+  if (e.f) return (float)((double)init + (double)e.v);
+  return (float)((pre.f ? (double)init : carry) + pre.v + (double)e.v);
 }
 
 template <int NT, int VT, bool inclusive, typename in_t, typename T, typename xform_t>
@@ -189,6 +210,7 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
   typedef typename traits::acc_t acc_t;
   typedef typename traits::carry_t carry_t;
   typedef seg_pair_t<acc_t> pair_t;
+  typedef seg_cpair_t<carry_t> cpair_t;
   constexpr int NW = NT / warp_size;
   constexpr int NV = NT * VT;
   constexpr int NR = VT / 4;
@@ -246,19 +268,16 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
   }
   __syncthreads();
 
-  pair_t run = {0u, acc_t(0)}, off[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    pair_t pre = run;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      pair_t e = {s_wave_f[r][w], s_wave_v[r][w]};
-      if (w < warp) pre = seg_combine(pre, e);
-      run = seg_combine(run, e);
+  // Tile aggregate as a (flag, carry-type) pair, combined by wave 0 over the
+  // NR x NW wave totals in order.
+  cpair_t aggregate = {0u, carry_t(0)};
+  if (warp == 0) {
+#pragma unroll 4
+    for (int i = 0; i < NR * NW; ++i) {
+      const int r = i / NW, w = i % NW;
+      aggregate = seg_combine_c(aggregate, s_wave_f[r][w], (carry_t)s_wave_v[r][w]);
     }
-    off[r] = pre;
   }
-  const pair_t aggregate = run;
 
   // A tile holding a head knows its carry-out, init + (sum after its last
   // head), without looking back and publishes it as inclusive at once; it
@@ -269,10 +288,9 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
     if (tile > 0) {
       if (lane == 0) {
         atomic_store_agent(status + tile,
-                           head_tile ? pack_status((carry_t)((carry_t)(acc_t)init +
-                                                             (carry_t)aggregate.v),
+                           head_tile ? pack_status((carry_t)((carry_t)(acc_t)init + aggregate.v),
                                                    status_inclusive)
-                                     : pack_status((carry_t)aggregate.v, status_aggregate));
+                                     : pack_status(aggregate.v, status_aggregate));
         // The word must be visible before this tile's output (possibly over
         // its own input) can be: wait for the store to complete.
         __builtin_amdgcn_s_waitcnt(0);
@@ -285,7 +303,7 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
       if (tile == 0 || !head_tile)
         atomic_store_agent(status + tile,
                            pack_status((carry_t)((head_tile ? (carry_t)(acc_t)init : carry) +
-                                                 (carry_t)aggregate.v),
+                                                 aggregate.v),
                                        status_inclusive));
       s_carry = carry;
     }
@@ -293,21 +311,31 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
   __syncthreads();
   const carry_t carry = s_carry;
 
+  // Everything above one wave row (the carry, earlier rows, earlier waves of
+  // this row) is combined in the carry type; inside a row the sums are acc_t.
   T o[VT];
+  cpair_t run = {0u, carry_t(0)};
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    const pair_t pre = seg_combine(off[r], w_excl[r]);
+    cpair_t pre = run;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t wf = s_wave_f[r][w];
+      const carry_t wv = (carry_t)s_wave_v[r][w];
+      if (w < warp) pre = seg_combine_c(pre, wf, wv);
+      run = seg_combine_c(run, wf, wv);
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      pair_t e;
+      pair_t e;  // in-row prefix pair of this element (acc_t)
       if (inclusive) {
-        e = seg_combine(pre, p[4 * r + q]);
+        e = seg_combine(w_excl[r], p[4 * r + q]);
       } else if (fl[4 * r + q]) {
         e = {1u, acc_t(0)};
       } else {
-        e = q == 0 ? pre : seg_combine(pre, p[4 * r + q - 1]);
+        e = q == 0 ? w_excl[r] : seg_combine(w_excl[r], p[4 * r + q - 1]);
       }
-      o[4 * r + q] = (T)segscan_finish(carry, (acc_t)init, e.f, e.v);
+      o[4 * r + q] = (T)segscan_finish(carry, (acc_t)init, pre, e);
     }
   }
   if (vec_ok && count - base >= NV) {
@@ -329,7 +357,7 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
   }
 }
 
-template <int NT = 256, int VT = 16>
+template <int NT = 256, int VT = 32>
 size_t segmented_scan_aux_bytes(int64_t count) {
   const int64_t num_tiles = div_up(count, (int64_t)(NT * VT));
   return aux_bytes<uint64_t>(num_tiles) + aux_bytes<int32_t>(num_tiles + 1);
@@ -339,7 +367,7 @@ size_t segmented_scan_aux_bytes(int64_t count) {
 // `init`.  segments: ascending starts, num_segments >= 0 (index 0 always
 // starts a segment; starts at or beyond count are empty trailing segments).
 // input/output may alias.
-template <int NT = 256, int VT = 16, typename in_t, typename T, typename xform_t>
+template <int NT = 256, int VT = 32, typename in_t, typename T, typename xform_t>
 mgpu_status_t segmented_transform_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
                                        const in_t* input, T* output, int64_t count,
                                        const int32_t* segments, int64_t num_segments,
@@ -384,7 +412,7 @@ mgpu_status_t segmented_transform_scan(void* aux_data, size_t& aux_size, hipStre
   return hip_status(e);
 }
 
-template <int NT = 256, int VT = 16, typename T>
+template <int NT = 256, int VT = 32, typename T>
 mgpu_status_t segmented_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
                              const T* input, T* output, int64_t count, const int32_t* segments,
                              int64_t num_segments, mgpu_scan_type_t type = MGPU_SCAN_EXCLUSIVE,

@@ -14,7 +14,7 @@ from mgpu_hip import workloads
 
 pytestmark = pytest.mark.gpu
 
-NV = 4096
+NV = 256 * 32  # elements per tile of the library's segmented scan
 F32_SCAN_RTOL = 2e-6
 SIZES = [1, 5, 4095, 4096, 4097, 3 * NV + 7, 1 << 20, (1 << 22) + 13]
 LAYOUTS = ["none", "single0", "every", "lognormal", "tile_edges", "sparse", "dupes"]
