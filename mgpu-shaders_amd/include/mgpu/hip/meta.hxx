@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -135,7 +136,7 @@ inline mgpu_status_t hip_status(hipError_t e) {
 
 struct timing_registry_t {
   std::mutex mutex;
-  bool enabled = false;
+  std::atomic<bool> enabled{false};
   struct rec_t { const char* name; hipEvent_t start, stop; };
   std::vector<rec_t> recs;
   std::vector<hipEvent_t> pool;
@@ -160,7 +161,9 @@ inline hipError_t launch_kernel(const char* name, kernel_t kernel, dim3 grid,
   if (grid.x == 0 || grid.y == 0 || grid.z == 0) return hipSuccess;
   timing_registry_t& tr = timing_registry();
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (tr.enabled) {
+  // Read once: a toggle from another thread must not split a launch's pair.
+  const bool timed = tr.enabled.load(std::memory_order_relaxed);
+  if (timed) {
     std::lock_guard<std::mutex> lock(tr.mutex);
     e0 = tr.get_event();
     e1 = tr.get_event();
@@ -168,7 +171,7 @@ inline hipError_t launch_kernel(const char* name, kernel_t kernel, dim3 grid,
   }
   hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
   hipError_t err = hipGetLastError();
-  if (tr.enabled) {
+  if (timed) {
     std::lock_guard<std::mutex> lock(tr.mutex);
     (void)hipEventRecord(e1, stream);
     tr.recs.push_back({name, e0, e1});
