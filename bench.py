@@ -192,6 +192,17 @@ def secondary(torch, mgpu_hip, steps, warmup):
         log(name, out[name]["keys_per_s"] / 1e9, "Gkeys/s, pass", per, "ms")
         del ks, vs, k, v, aux
 
+    # 2^20 uint32 keys, key-only mergesort (a latency-sized case: fits in L2 / MALL).
+    c1 = 1 << 20
+    ks = gen(c1, "u32", 3)
+    k = torch.empty_like(ks)
+    aux = mgpu_hip.mergesort_keys(k)
+    ms, _ = timed_steps(torch, steps, warmup, lambda: k.copy_(ks),
+                        lambda: mgpu_hip.mergesort_keys(k, aux=aux))
+    out["mergesort_keys_u32_2p20"] = {"keys_per_s": c1 / (ms / 1e3), "ms": ms}
+    log("mergesort_keys_u32_2p20", out["mergesort_keys_u32_2p20"])
+    del ks, k, aux
+
     # Merge of two sorted 2^27 uint32 arrays, key-only.
     m = 1 << 27
     a, b = gen(m, "u32", 1), gen(m, "u32", 2)
