@@ -272,6 +272,8 @@ mgpu_status_t mgpu_server_drain(mgpu_server_t server);
 mgpu_status_t mgpu_server_release(mgpu_server_t server, int64_t id);
 void mgpu_server_stats(mgpu_server_t server, size_t* in_use, size_t* peak, int64_t* pending,
                        int64_t* in_flight);
+/* Waits for the requests in flight, drops those not yet started, frees all
+ * device buffers the server holds.                                        */
 void mgpu_server_destroy(mgpu_server_t server);
 
 /* ------------------------------------------------------------------------ */

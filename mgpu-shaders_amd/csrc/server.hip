@@ -240,6 +240,10 @@ void worker_loop(mgpu_server_st* sv) {
   std::unique_lock<std::mutex> lock(sv->m);
   while (true) {
     bool progressed = false;
+    // Destroy: requests not yet started are dropped (their records are freed
+    // by destroy); only those in flight are waited for.  Without this a
+    // pending request that no longer fits the budget would never leave.
+    if (sv->stop) sv->pending.clear();
     // Completions.
     for (size_t i = 0; i < sv->slot_rec.size(); ++i) {
       rec_t* r = sv->slot_rec[i];

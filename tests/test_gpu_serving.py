@@ -125,3 +125,23 @@ def test_serving_without_kept_inputs():
         sv.release(rid)
     finally:
         sv.close()
+
+
+def test_serving_destroy_with_blocked_pending_requests():
+    """Results held unreleased fill the budget, so later requests can never be
+    admitted; destroying the server must still return (dropping them)."""
+    banner("serving destroy with blocked pending")
+    import time
+    n = 1 << 20
+    one = serving.request_bytes("sort", "u32", n)
+    sv = serving.Server(slots=4, budget_bytes=one + one // 2)
+    ids = [sv.submit("sort", "u32", n, 70 + i) for i in range(4)]
+    rc = sv.lib.mgpu_server_drain(sv.h)
+    assert rc == 2  # MGPU_AUX_TOO_SMALL: one result holds the budget, three wait
+    got = sv.poll()
+    assert len(got) == 1 and got[0]["id"] == ids[0]
+    serving.verify(got[0])
+    assert sv.stats()["pending"] == 3
+    t0 = time.time()
+    sv.close()
+    assert time.time() - t0 < 10
