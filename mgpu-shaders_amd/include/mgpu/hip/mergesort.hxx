@@ -68,13 +68,14 @@ struct blocksort_storage_t {
 
 // Stable odd-even transposition sort of VT registers; bit j of `oor` marks
 // slot j as out of range (it never moves and nothing swaps with it).
-template <int VT, bool has_values, typename key_t, typename val_t, typename comp_t>
+template <int VT, bool has_values, bool masked = true, typename key_t, typename val_t,
+          typename comp_t>
 __device__ inline void odd_even_sort(key_t (&k)[VT], val_t (&v)[VT], uint32_t oor, comp_t comp) {
 #pragma unroll
   for (int I = 0; I < VT; ++I) {
 #pragma unroll
     for (int i = 1 & I; i < VT - 1; i += 2) {
-      if (!((oor >> (i + 1)) & 1) && comp(k[i + 1], k[i])) {
+      if ((!masked || !((oor >> (i + 1)) & 1)) && comp(k[i + 1], k[i])) {
         key_t t = k[i]; k[i] = k[i + 1]; k[i + 1] = t;
         if constexpr (has_values) { val_t u = v[i]; v[i] = v[i + 1]; v[i + 1] = u; }
       }
@@ -160,7 +161,9 @@ __device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in,
     for (int i = 0; i < VT; ++i)
       if (i >= valid) oor |= 1u << i;
   }
-  odd_even_sort<VT, has_values>(k, v, oor, comp);
+  // Full tiles (every thread's slots in range) skip the mask test.
+  if (tile_count == NV) odd_even_sort<VT, has_values, false>(k, v, 0u, comp);
+  else odd_even_sort<VT, has_values, true>(k, v, oor, comp);
 #pragma unroll 1
   for (int pass = 0; (1 << pass) < NT; ++pass) {
     if ((VT << pass) >= tile_count) break;  // one sorted run covers the tile
