@@ -305,6 +305,14 @@ mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int 
  * Tests set a tiny limit to force that path; results are identical.
  * mgpu_lookback_fallbacks returns how often it was taken on the current
  * device (synchronising the device) and optionally resets the count.       */
+/* Digit-pass structure of the radix sort (process-wide): 0 = automatic
+ * (single pass up to 2^22 keys, count + scan + scatter above), 1 = per-pass
+ * count + scan + scatter, 2 = single pass (all-pass histogram, then one
+ * look-back kernel per pass) wherever count < 2^30.  Results are identical;
+ * the sizing call returns enough aux for either.  No counterpart in the
+ * reference (vk/radix.hxx has one structure).                               */
+void mgpu_set_radix_algorithm(int algorithm);
+int mgpu_radix_algorithm(void);
 void mgpu_set_lookback_spin_limit(uint32_t limit);
 uint32_t mgpu_lookback_spin_limit(void);
 int64_t mgpu_lookback_fallbacks(int reset);

@@ -26,3 +26,8 @@ MGPU_DEF_RADIX(u64, uint64_t)
 MGPU_DEF_RADIX(f32, float)
 MGPU_DEF_RADIX(i64, int64_t)
 MGPU_DEF_RADIX(f64, double)
+
+extern "C" void mgpu_set_radix_algorithm(int algorithm) {
+  radix_algorithm_setting().store(algorithm < 0 || algorithm > 2 ? 0 : algorithm);
+}
+extern "C" int mgpu_radix_algorithm(void) { return radix_algorithm_setting().load(); }

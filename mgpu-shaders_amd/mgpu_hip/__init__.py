@@ -116,6 +116,9 @@ def load():
     lib.mgpu_graph_destroy.argtypes = [vp]
     lib.mgpu_generate.restype = c_int
     lib.mgpu_generate.argtypes = [vp, vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
+    lib.mgpu_set_radix_algorithm.restype = None
+    lib.mgpu_set_radix_algorithm.argtypes = [c_int]
+    lib.mgpu_radix_algorithm.restype = c_int
     lib.mgpu_set_lookback_spin_limit.restype = None
     lib.mgpu_set_lookback_spin_limit.argtypes = [ctypes.c_uint32]
     lib.mgpu_lookback_spin_limit.restype = ctypes.c_uint32
@@ -430,6 +433,18 @@ def generate(out, kind, seed=0, offset=0, stream=None):
     _check(load().mgpu_generate(_stream(stream), _ptr(out), out.numel(), out.element_size(),
                                 GEN_KINDS[kind], seed, offset), "generate")
     return out
+
+
+RADIX_AUTO, RADIX_COUNT_SCAN_SCATTER, RADIX_SINGLE_PASS = 0, 1, 2
+
+
+def set_radix_algorithm(algorithm):
+    """0 = automatic, 1 = count + scan + scatter per pass, 2 = single pass."""
+    load().mgpu_set_radix_algorithm(int(algorithm))
+
+
+def radix_algorithm():
+    return int(load().mgpu_radix_algorithm())
 
 
 def set_lookback_spin_limit(limit):

@@ -78,7 +78,10 @@ def test_sizing_radix_and_scan():
     tiles = -(-n // (512 * 20))
     assert st == 0
     scan_aux = r256(8 * (-(-tiles * 256 // 4096)))  # counts scan: 4096-element tiles
-    assert sz == r256(4 * n) * 2 + r256(4 * tiles * 256) + scan_aux
+    scan_path = r256(4 * tiles * 256) + scan_aux
+    # Single pass: [4 passes][tiles][256] status words, all-pass counts, bases.
+    single_pass = r256(4 * tiles * 256 * 4) + 2 * r256(4 * 4 * 256)
+    assert sz == r256(4 * n) * 2 + max(scan_path, single_pass)
     st, sz = _size(lib.mgpu_scan_exclusive_i32, ctypes.c_void_p(1), n, 0)
     assert st == 0 and sz == r256(8 * -(-n // (256 * 80)))  # 8 B per 20480-element tile
 

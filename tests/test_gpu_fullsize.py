@@ -14,14 +14,20 @@ N_SORT = 1 << 28
 
 
 @pytest.mark.parametrize("kind", ["u32", "lowentropy"])
-@pytest.mark.parametrize("algo", ["radix", "mergesort"])
+@pytest.mark.parametrize("algo", ["radix_count_scan_scatter", "radix_single_pass", "mergesort"])
 def test_sort_2p28_kv(kind, algo):
     banner("fullsize sort", algo, kind)
     keys_in = gen_dev(N_SORT, kind, 0)
     keys = keys_in.clone()
     vals = gen_dev(N_SORT, "iota")
-    if algo == "radix":
-        mgpu_hip.radix_sort(keys, vals)
+    if algo.startswith("radix"):
+        old = mgpu_hip.radix_algorithm()
+        mgpu_hip.set_radix_algorithm(mgpu_hip.RADIX_SINGLE_PASS if algo == "radix_single_pass"
+                                     else mgpu_hip.RADIX_COUNT_SCAN_SCATTER)
+        try:
+            mgpu_hip.radix_sort(keys, vals)
+        finally:
+            mgpu_hip.set_radix_algorithm(old)
     else:
         mgpu_hip.mergesort_kv(keys, vals)
     torch.cuda.synchronize()

@@ -39,6 +39,16 @@ def dev_keys(x):
     return to_dev(x)
 
 
+@pytest.fixture(params=[mgpu_hip.RADIX_COUNT_SCAN_SCATTER, mgpu_hip.RADIX_SINGLE_PASS],
+                ids=["count_scan_scatter", "single_pass"])
+def radix_algo(request):
+    """Runs the test under each digit-pass structure of the radix sort."""
+    old = mgpu_hip.radix_algorithm()
+    mgpu_hip.set_radix_algorithm(request.param)
+    yield request.param
+    mgpu_hip.set_radix_algorithm(old)
+
+
 # ------------------------------------------------------------------ mergesort
 
 @pytest.mark.parametrize("n", SIZES)
@@ -155,8 +165,8 @@ def test_merge_vk_merge_known_answer():
 
 @pytest.mark.parametrize("n", SIZES + [3 * RADIX_NV, 3 * RADIX_NV + 5])
 @pytest.mark.parametrize("kt", KTS)
-def test_radix_kv(n, kt):
-    banner("radix_kv", kt, n)
+def test_radix_kv(n, kt, radix_algo):
+    banner("radix_kv", kt, n, radix_algo)
     keys = make_keys(kt, n, 8, "ties")
     vals = np.arange(n, dtype=np.uint32)
     dk, dv = dev_keys(keys), to_dev(vals)
@@ -168,9 +178,9 @@ def test_radix_kv(n, kt):
 
 @pytest.mark.parametrize("pattern", ["random", "equal", "sorted", "reversed", "lowentropy"])
 @pytest.mark.parametrize("kt", ["u32", "u64"])
-def test_radix_patterns(pattern, kt):
+def test_radix_patterns(pattern, kt, radix_algo):
     n = 5 * RADIX_NV + 77
-    banner("radix pattern", pattern, kt)
+    banner("radix pattern", pattern, kt, radix_algo)
     keys = make_keys(kt, n, 9, pattern)
     vals = np.arange(n, dtype=np.uint32)
     dk, dv = dev_keys(keys), to_dev(vals)
@@ -180,8 +190,8 @@ def test_radix_patterns(pattern, kt):
     np.testing.assert_array_equal(to_host(dv), ev)
 
 
-def test_radix_special_floats():
-    banner("radix special floats")
+def test_radix_special_floats(radix_algo):
+    banner("radix special floats", radix_algo)
     base = np.array([np.nan, np.inf, 1.0, 0.0, -0.0, -1.0, -np.inf, 1e-45, -1e-45, 3.4e38,
                      -3.4e38], dtype=np.float32)
     neg_nan = np.array([0xFFC00000], dtype=np.uint32).view(np.float32)
@@ -194,8 +204,8 @@ def test_radix_special_floats():
     np.testing.assert_array_equal(to_host(dv), ev)
 
 
-def test_radix_special_doubles_and_int64_extremes():
-    banner("radix special doubles, int64 extremes")
+def test_radix_special_doubles_and_int64_extremes(radix_algo):
+    banner("radix special doubles, int64 extremes", radix_algo)
     base = np.array([np.nan, np.inf, 1.0, 0.0, -0.0, -1.0, -np.inf, 5e-324, -5e-324, 1.7e308,
                      -1.7e308], dtype=np.float64)
     neg_nan = np.array([0xFFF8000000000000], dtype=np.uint64).view(np.float64)
@@ -216,7 +226,7 @@ def test_radix_special_doubles_and_int64_extremes():
     np.testing.assert_array_equal(to_host(dv), ev)
 
 
-def test_radix_keys_vk_radix2_known_answer():
+def test_radix_keys_vk_radix2_known_answer(radix_algo):
     # examples/vk_radix2/vk_radix2.cxx:14-75: 8,388,608 glibc rand() keys,
     # sorted six times in place, compared with std::sort.
     banner("vk_radix2 kat")
@@ -229,8 +239,8 @@ def test_radix_keys_vk_radix2_known_answer():
     np.testing.assert_array_equal(to_host(dk), oracle.std_sort_u32(keys))
 
 
-def test_radix_repeat_is_deterministic_and_idempotent():
-    banner("radix repeat")
+def test_radix_repeat_is_deterministic_and_idempotent(radix_algo):
+    banner("radix repeat", radix_algo)
     n = (1 << 21) + 5
     keys = gen_dev(n, "lowentropy", 10)
     vals = gen_dev(n, "iota")
@@ -248,10 +258,11 @@ def test_radix_repeat_is_deterministic_and_idempotent():
 
 @pytest.mark.parametrize("limit", [0, 1])
 @pytest.mark.parametrize("kt", ["u32", "u64", "f32"])
-def test_radix_forced_lookback_fallback(limit, kt):
-    """A tiny spin limit makes tiles count late predecessors from their input;
-    the sort must be unchanged."""
-    banner("radix fallback", kt, limit)
+def test_radix_forced_lookback_fallback(limit, kt, radix_algo):
+    """A tiny spin limit makes tiles count late predecessors from their input
+    (the scan's tile sums, or the single pass's per-digit counts); the sort
+    must be unchanged."""
+    banner("radix fallback", kt, limit, radix_algo)
     n = 40 * RADIX_NV + 123
     keys = make_keys(kt, n, 12, "ties")
     vals = np.arange(n, dtype=np.uint32)
