@@ -203,6 +203,33 @@ def secondary(torch, mgpu_hip, steps, warmup):
     log("mergesort_keys_u32_2p20", out["mergesort_keys_u32_2p20"])
     del ks, k, aux
 
+    # Key-only uint32 radix sort at the sizes of the reference's only published
+    # benchmark (examples/vk_radix/vk_radix.cxx:40-71: 10^6 x s keys), next to
+    # the chart's readings for it (examples/vk_radix/timings.jpg, RTX 2060,
+    # Mkeys/s read off the chart as a range).  Keys are restored to uniform
+    # random before every sort; each sort is timed alone with events.
+    chart = {1: (2530, 2630), 2: (3570, 3670), 4: (3960, 4060), 6: (3960, 4060),
+             8: (3960, 4060), 50: (3850, 3950)}
+    rows = []
+    for s_m in (1, 2, 4, 6, 8, 10, 12, 14, 16, 20, 25, 30, 35, 40, 45, 50):
+        cnt = 1000000 * s_m
+        ks = gen(cnt, "u32", 4)
+        k = torch.empty_like(ks)
+        aux = mgpu_hip.radix_sort(k)
+        ms, _ = timed_steps(torch, max(steps, 20), warmup, lambda: k.copy_(ks),
+                            lambda: mgpu_hip.radix_sort(k, aux=aux))
+        lo, hi = chart.get(s_m, (4110, 4200))
+        rows.append({"keys": cnt, "ms": ms, "mkeys_per_s": cnt / (ms / 1e3) / 1e6,
+                     "chart_mkeys_per_s": [lo, hi],
+                     "ratio_vs_chart_high": cnt / (ms / 1e3) / 1e6 / hi})
+        del ks, k, aux
+    out["radix_keys_u32_reference_chart_sizes"] = {
+        "rows": rows, "algorithm": "automatic (single pass up to 2^22 keys)",
+        "min_ratio_vs_chart_high": min(r["ratio_vs_chart_high"] for r in rows)}
+    log("radix key-only at chart sizes",
+        [(r["keys"] // 1000000, round(r["mkeys_per_s"]), round(r["ratio_vs_chart_high"], 1))
+         for r in rows])
+
     # Merge of two sorted 2^27 uint32 arrays, key-only.
     m = 1 << 27
     a, b = gen(m, "u32", 1), gen(m, "u32", 2)

@@ -16,12 +16,12 @@
 //      of digit d: sum_{d'<d} total(d') + sum_{t'<t} count(t', d), the
 //      (bin, tile) order of vk/radix.hxx:357-375;
 //   3. kernel_radix_scatter: ranks every key within its tile (wave64 match
-//      through LDS atomic OR on 64-bit lane masks + per-wave counters),
+//      from 8 ballots on 64-bit lane masks + per-wave counters in LDS),
 //      reorders the tile by digit in LDS, and writes keys and values in runs
 //      of consecutive addresses (16 bytes moved per key-value pair).
-// A single-pass (onesweep) variant was measured slower on MI355X: each tile
-// would have to sum ~100 predecessor tiles x 256 digits through cross-XCD
-// status words, and the look-back latency dominated the pass.
+// Up to 2^22 keys a single-pass structure is faster (fewer launches; see
+// kernel_radix_onesweep); above that its look-back over ~10^4 tiles through
+// cross-XCD status words dominates the pass.
 // Tiles are remapped so that each XCD (blockIdx % 8) owns a contiguous tile
 // range, which keeps the count/offset lines a tile touches in its XCD's L2.
 // Digits go from least to most significant and every pass is stable, so the
@@ -119,10 +119,9 @@ struct radix_scatter_storage_t {
     key_t keys[NT * VT];                   // tile reordered by digit (then values)
     uint64_t match[NW][radix_num_bins];    // per-wave lane masks of each digit (ranking)
   };
-  uint32_t hist[NW][radix_num_bins];       // per-wave counts -> wave prefixes
+  uint32_t hist[NW][radix_num_bins];       // per-wave counts -> tile positions of wave runs
   uint32_t hist_sink[NW][warp_size];       // write target of lanes that update nothing
   uint64_t match_sink[NW][warp_size];
-  uint32_t local_off[radix_num_bins];      // tile-local exclusive digit offsets
   uint32_t global_off[radix_num_bins];     // output position of local slot 0 per digit
   uint32_t scan_tot[NW];
 };
@@ -161,6 +160,7 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   static_assert(sizeof(key_t) * NT * VT >= sizeof(uint64_t) * NW * radix_num_bins,
                 "match table must fit in the key buffer");
   static_assert(NT * VT <= 65536, "ranks are packed in 16 bits");
+  constexpr bool ballot_rank = sizeof(key_t) == 8;
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int warp = tid / warp_size;
@@ -180,43 +180,62 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
                ? ((uint32_t)(xb(k[i]) >> shift) & 255u)
                : 255u;
   };
-  // Wave-private counters and match table (LDS operations of one wave are
-  // performed in order, so no block barrier is needed before use).
+  // Wave-private counters (and match table) in LDS: the operations of one
+  // wave are performed in order, so no block barrier is needed before use.
 #pragma unroll
-  for (int b = lane; b < radix_num_bins; b += warp_size) {
-    s.hist[warp][b] = 0;
-    s.match[warp][b] = 0;
+  for (int j = 0; j < radix_num_bins / warp_size; ++j) {
+    s.hist[warp][lane + j * warp_size] = 0;
+    if constexpr (!ballot_rank) s.match[warp][lane + j * warp_size] = 0;
   }
 
-  // Wave-local stable rank: the lanes sharing the first lane's digit get
-  // their mask from one ballot; every other lane ORs its bit into the table
-  // entry of its digit and reads the entry back.  The lowest lane of each
-  // digit advances the wave's running count and clears the entry; the other
-  // lanes write the same values to private sink slots, so the writes need no
-  // branch.
+  // Wave-local stable rank: rank = the wave's running count of the digit
+  // (per-wave counters in LDS) + the number of lower lanes with the same
+  // digit in this slot.  The lowest lane of each digit advances the count;
+  // the other lanes write the same value to a private sink slot, so the write
+  // needs no branch.  The mask of lanes with the same digit comes from
+  //  * 32-bit keys: the lanes sharing the first lane's digit from one ballot;
+  //    every other lane ORs its bit into the table entry of its digit, reads
+  //    the entry back, and the lowest lane clears it;
+  //  * 64-bit keys: the AND over the digit's 8 bits of the ballot of that bit
+  //    (or of its complement), in registers: these tiles stage 8-byte keys
+  //    through LDS, so LDS is the tighter resource.
+  // (Each was measured faster than the other for its key size on MI355X.)
   const uint64_t lane_bit = 1ull << lane;
   uint32_t r[VT];
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     const uint32_t d = digit(i);
-    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-    const uint64_t same = __ballot(d == d0);
-    const bool tab = d != d0;
     // The counter read goes first so that it is in flight together with the
-    // table read (one LDS round trip per slot).
+    // table read or the ballots (one LDS round trip per slot).
     const uint32_t c = s.hist[warp][d];
-    uint64_t m = same;
-    if (tab) {
-      atomicOr((unsigned long long*)&s.match[warp][d], (unsigned long long)lane_bit);
-      m = s.match[warp][d];
+    uint64_t m;
+    bool clear = false;
+    if constexpr (ballot_rank) {
+      m = ~0ull;
+#pragma unroll
+      for (int b = 0; b < radix_num_bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t ones = __ballot(bit);
+        m &= bit ? ones : ~ones;
+      }
+    } else {
+      const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+      m = __ballot(d == d0);
+      clear = d != d0;
+      if (clear) {
+        atomicOr((unsigned long long*)&s.match[warp][d], (unsigned long long)lane_bit);
+        m = s.match[warp][d];
+      }
     }
     const uint32_t below = mask_count_below(m);
     r[i] = (d << 16) | (c + below);  // the digit rides along for the reorder
     const bool lead = below == 0;
     uint32_t* hp = lead ? &s.hist[warp][d] : &s.hist_sink[warp][lane];
     *hp = c + (uint32_t)__popcll(m);
-    uint64_t* mp = lead && tab ? &s.match[warp][d] : &s.match_sink[warp][lane];
-    *mp = 0ull;
+    if constexpr (!ballot_rank) {
+      uint64_t* mp = lead && clear ? &s.match[warp][d] : &s.match_sink[warp][lane];
+      *mp = 0ull;
+    }
   }
   __syncthreads();
 
@@ -241,7 +260,9 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   if (tid < radix_num_bins) {
     off_local = incl - total;
     for (int w = 0; w < warp; ++w) off_local += s.scan_tot[w];
-    s.local_off[tid] = off_local;
+    // hist[w][d] becomes the tile position of wave w's first key of digit d.
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s.hist[w][tid] += off_local;
   }
   __syncthreads();
 
@@ -263,7 +284,7 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     const uint32_t d = r[i] >> 16;
-    const uint32_t pos = s.local_off[d] + s.hist[warp][d] + (r[i] & 0xFFFFu);
+    const uint32_t pos = s.hist[warp][d] + (r[i] & 0xFFFFu);
     s.keys[pos] = k[i];
     r[i] = pos;
   }
