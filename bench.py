@@ -20,10 +20,14 @@ The JSON line also carries:
   segsort       65536 lognormal segments (~2^25.6 keys): batched segmented
                 sort (eager and graph replay) vs a loop of plain sorts;
   secondary     the other primitives at their workload sizes (mergesort kv
-                2^28, merge 2 x 2^27, scans / reduce 2^29, low-entropy radix).
+                2^28, merge 2 x 2^27, scans / reduce 2^29, low-entropy radix,
+                segmented scan), and key-only radix at the sizes of the
+                reference's published radix chart (10^6 x {1 ... 50} keys);
+  serving       the request-serving engine at saturation and three loads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
-                       [--no-cpu] [--cpu-log2 L]
+                       [--no-cpu] [--no-serving] [--no-segsort] [--no-chart]
+                       [--cpu-log2 L]
 """
 import argparse
 import json
@@ -154,7 +158,7 @@ def cpu_baselines_other(log2n=27):
     return out
 
 
-def secondary(torch, mgpu_hip, steps, warmup):
+def secondary(torch, mgpu_hip, steps, warmup, chart=True):
     out = {}
     dev = "cuda"
     n = 1 << 28
@@ -211,7 +215,7 @@ def secondary(torch, mgpu_hip, steps, warmup):
     chart = {1: (2530, 2630), 2: (3570, 3670), 4: (3960, 4060), 6: (3960, 4060),
              8: (3960, 4060), 50: (3850, 3950)}
     rows = []
-    for s_m in (1, 2, 4, 6, 8, 10, 12, 14, 16, 20, 25, 30, 35, 40, 45, 50):
+    for s_m in (1, 2, 4, 6, 8, 10, 12, 14, 16, 20, 25, 30, 35, 40, 45, 50) if chart else ():
         cnt = 1000000 * s_m
         ks = gen(cnt, "u32", 4)
         k = torch.empty_like(ks)
@@ -223,12 +227,13 @@ def secondary(torch, mgpu_hip, steps, warmup):
                      "chart_mkeys_per_s": [lo, hi],
                      "ratio_vs_chart_high": cnt / (ms / 1e3) / 1e6 / hi})
         del ks, k, aux
-    out["radix_keys_u32_reference_chart_sizes"] = {
-        "rows": rows, "algorithm": "automatic (single pass up to 2^22 keys)",
-        "min_ratio_vs_chart_high": min(r["ratio_vs_chart_high"] for r in rows)}
-    log("radix key-only at chart sizes",
-        [(r["keys"] // 1000000, round(r["mkeys_per_s"]), round(r["ratio_vs_chart_high"], 1))
-         for r in rows])
+    if rows:
+        out["radix_keys_u32_reference_chart_sizes"] = {
+            "rows": rows, "algorithm": "automatic (single pass up to 2^22 keys)",
+            "min_ratio_vs_chart_high": min(r["ratio_vs_chart_high"] for r in rows)}
+        log("radix key-only at chart sizes",
+            [(r["keys"] // 1000000, round(r["mkeys_per_s"]), round(r["ratio_vs_chart_high"], 1))
+             for r in rows])
 
     # Merge of two sorted 2^27 uint32 arrays, key-only.
     m = 1 << 27
@@ -418,6 +423,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-segsort", action="store_true")
+    ap.add_argument("--no-chart", action="store_true",
+                    help="skip the key-only radix sizes of the reference chart")
     ap.add_argument("--cpu-log2", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -492,7 +499,8 @@ def main():
         if not args.no_segsort:
             result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
             torch.cuda.empty_cache()
-        result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
+        result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2,
+                                        chart=not args.no_chart)
         torch.cuda.empty_cache()
         if not args.no_serving:
             result["serving"] = bench_serving(torch, mgpu_hip)
