@@ -107,7 +107,9 @@ MGPU_DECL_MERGE(f64, double)
  * value order; signed: MSB flipped; float: -0.0 before +0.0, NaNs last),
  * stable, in place.  Replaces vk::radix_sort (inc/mgpu/vk/radix.hxx:195-470)
  * and extends it with values.  Bit order per inc/mgpu/common/cta_radix.hxx:
- * 6-64.  count < 2^32 - 10240 (32-bit positions).                          */
+ * 6-64.  count < 2^32 - 12288 (32-bit positions, less one tile).  The tile
+ * shape is chosen by key size, values and count (small sorts use small
+ * tiles); aux sizing follows the same choice.                              */
 #define MGPU_DECL_RADIX(K, KT)                                                 \
   mgpu_status_t mgpu_radix_sort_keys_##K(void* aux, size_t* aux_size,          \
       mgpu_stream_t stream, KT* keys, int64_t count);                          \
@@ -306,11 +308,11 @@ mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int 
  * mgpu_lookback_fallbacks returns how often it was taken on the current
  * device (synchronising the device) and optionally resets the count.       */
 /* Digit-pass structure of the radix sort (process-wide): 0 = automatic
- * (single pass up to 2^22 keys, count + scan + scatter above), 1 = per-pass
- * count + scan + scatter, 2 = single pass (all-pass histogram, then one
- * look-back kernel per pass) wherever count < 2^30.  Results are identical;
- * the sizing call returns enough aux for either.  No counterpart in the
- * reference (vk/radix.hxx has one structure).                               */
+ * (count + scan + scatter, the faster structure on MI355X at every size),
+ * 1 = per-pass count + scan + scatter, 2 = single pass (all-pass histogram,
+ * then one look-back kernel per pass) wherever count < 2^30.  Results are
+ * identical; the sizing call returns enough aux for either.  No counterpart
+ * in the reference (vk/radix.hxx has one structure).                        */
 void mgpu_set_radix_algorithm(int algorithm);
 int mgpu_radix_algorithm(void);
 void mgpu_set_lookback_spin_limit(uint32_t limit);

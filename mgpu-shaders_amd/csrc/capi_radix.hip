@@ -1,6 +1,5 @@
 // C ABI: LSD radix sort (inc/mgpu/vk/radix.hxx:195-470 counterpart, plus values).
-#include "mgpu/hip/radix.hxx"
-#include "tuning.hxx"
+#include "radix_dispatch.hxx"
 
 using namespace mgpu::hip;
 
@@ -9,15 +8,14 @@ using namespace mgpu::hip;
                                                     mgpu_stream_t stream, KT* keys,         \
                                                     int64_t count) {                        \
     if (!aux_size) return MGPU_INVALID_ARG;                                                 \
-    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt, CNT = radix_tuning<KT>::cnt;                     \
-    return radix_sort<NT, VT, CNT>(aux, *aux_size, (hipStream_t)stream, keys, count);            \
+    return radix_sort_tuned(aux, *aux_size, (hipStream_t)stream, keys, (empty_t*)nullptr,      \
+                            count);                                                         \
   }                                                                                         \
   extern "C" mgpu_status_t mgpu_radix_sort_kv_##K##_u32(void* aux, size_t* aux_size,        \
                                                         mgpu_stream_t stream, KT* keys,     \
                                                         uint32_t* vals, int64_t count) {    \
     if (!aux_size) return MGPU_INVALID_ARG;                                                 \
-    constexpr int NT = radix_tuning<KT>::nt, VT = radix_tuning<KT>::vt, CNT = radix_tuning<KT>::cnt;                     \
-    return radix_sort<NT, VT, CNT>(aux, *aux_size, (hipStream_t)stream, keys, vals, count);      \
+    return radix_sort_tuned(aux, *aux_size, (hipStream_t)stream, keys, vals, count);          \
   }
 
 MGPU_DEF_RADIX(u32, uint32_t)

@@ -21,8 +21,8 @@
 
 #include "mgpu/hip/generate.hxx"
 #include "mgpu/hip/merge.hxx"
-#include "mgpu/hip/radix.hxx"
 #include "mgpu/hip/scan.hxx"
+#include "radix_dispatch.hxx"
 #include "tuning.hxx"
 
 using namespace mgpu::hip;
@@ -96,9 +96,9 @@ size_t request_bytes(const mgpu_request_t& r) {
   switch (r.op) {
     case MGPU_REQ_SORT:
       if (r.key_type == MGPU_KEY_U64)
-        radix_sort<radix_tuning<uint64_t>::nt, radix_tuning<uint64_t>::vt, radix_tuning<uint64_t>::cnt>(nullptr, aux, nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
+        radix_sort_tuned(nullptr, aux, nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
       else
-        radix_sort<radix_tuning<uint32_t>::nt, radix_tuning<uint32_t>::vt, radix_tuning<uint32_t>::cnt>(nullptr, aux, nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
+        radix_sort_tuned(nullptr, aux, nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
       return 2 * aux_round(n * kb) + aux_round(n * 4) + aux_round(aux);
     case MGPU_REQ_MERGE:
       if (r.key_type == MGPU_KEY_U64)
@@ -142,9 +142,9 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
     case MGPU_REQ_SORT: {
       const int kind = q.key_type == MGPU_KEY_U64 ? gen_u64 : q.key_type == MGPU_KEY_F32 ? gen_f32_pm1 : gen_u32;
       if (q.key_type == MGPU_KEY_U64)
-        radix_sort<radix_tuning<uint64_t>::nt, radix_tuning<uint64_t>::vt, radix_tuning<uint64_t>::cnt>(nullptr, aux, s, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
+        radix_sort_tuned(nullptr, aux, s, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
       else
-        radix_sort<radix_tuning<uint32_t>::nt, radix_tuning<uint32_t>::vt, radix_tuning<uint32_t>::cnt>(nullptr, aux, s, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
+        radix_sort_tuned(nullptr, aux, s, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
       if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->out, n * kb, s)) ||
           !ok(alloc(r, &r->vals, n * 4, s)) || !ok(alloc(r, &r->aux, aux, s)))
         break;
@@ -156,16 +156,16 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
       mgpu_status_t st;
       switch (q.key_type) {
         case MGPU_KEY_U64:
-          st = radix_sort<radix_tuning<uint64_t>::nt, radix_tuning<uint64_t>::vt, radix_tuning<uint64_t>::cnt>(r->aux, cap, s, (uint64_t*)r->out, (uint32_t*)r->vals, n);
+          st = radix_sort_tuned(r->aux, cap, s, (uint64_t*)r->out, (uint32_t*)r->vals, n);
           break;
         case MGPU_KEY_I32:
-          st = radix_sort<radix_tuning<uint32_t>::nt, radix_tuning<uint32_t>::vt, radix_tuning<uint32_t>::cnt>(r->aux, cap, s, (int32_t*)r->out, (uint32_t*)r->vals, n);
+          st = radix_sort_tuned(r->aux, cap, s, (int32_t*)r->out, (uint32_t*)r->vals, n);
           break;
         case MGPU_KEY_F32:
-          st = radix_sort<radix_tuning<uint32_t>::nt, radix_tuning<uint32_t>::vt, radix_tuning<uint32_t>::cnt>(r->aux, cap, s, (float*)r->out, (uint32_t*)r->vals, n);
+          st = radix_sort_tuned(r->aux, cap, s, (float*)r->out, (uint32_t*)r->vals, n);
           break;
         default:
-          st = radix_sort<radix_tuning<uint32_t>::nt, radix_tuning<uint32_t>::vt, radix_tuning<uint32_t>::cnt>(r->aux, cap, s, (uint32_t*)r->out, (uint32_t*)r->vals, n);
+          st = radix_sort_tuned(r->aux, cap, s, (uint32_t*)r->out, (uint32_t*)r->vals, n);
       }
       return st;
     }

@@ -10,11 +10,6 @@ template <typename key_t, int bytes = sizeof(key_t)> struct merge_tuning { stati
 template <typename key_t> struct merge_tuning<key_t, 8> { static constexpr int nt = 256, vt = 15; };
 template <typename key_t, int bytes = sizeof(key_t)> struct mergesort_tuning { static constexpr int nt = 512, vt = 19; };
 template <typename key_t> struct mergesort_tuning<key_t, 8> { static constexpr int nt = 256, vt = 15; };
-// Radix: scatter block nt x vt (tile = nt*vt keys), count block cnt threads.
-// 512 x 20 (10240-key tiles) measured fastest for 2^28 u32 key-value on
-// MI355X (4.36 ms; 512 x 24: 4.47, 512 x 16: 4.54, 768 x 16: 4.68).
-template <typename key_t, int bytes = sizeof(key_t)> struct radix_tuning { static constexpr int nt = 512, vt = 20, cnt = 512; };
-template <typename key_t> struct radix_tuning<key_t, 8> { static constexpr int nt = 512, vt = 12, cnt = 512; };
 // Scan: 256 x 80 (20480-element tiles).  Larger tiles mean fewer look-back
 // steps per byte; 2^29 int32 exclusive: 0.775 ms at 256 x 80 vs 1.19 ms at
 // 256 x 16 on MI355X (float32 inclusive 0.822 vs 1.26 ms).

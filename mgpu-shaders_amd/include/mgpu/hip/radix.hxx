@@ -19,9 +19,8 @@
 //      from 8 ballots on 64-bit lane masks + per-wave counters in LDS),
 //      reorders the tile by digit in LDS, and writes keys and values in runs
 //      of consecutive addresses (16 bytes moved per key-value pair).
-// Up to 2^22 keys a single-pass structure is faster (fewer launches; see
-// kernel_radix_onesweep); above that its look-back over ~10^4 tiles through
-// cross-XCD status words dominates the pass.
+// A single-pass structure (kernel_radix_onesweep) is available by setting;
+// on MI355X it is slower at every size (see radix_algorithm_setting).
 // Tiles are remapped so that each XCD (blockIdx % 8) owns a contiguous tile
 // range, which keeps the count/offset lines a tile touches in its XCD's L2.
 // Digits go from least to most significant and every pass is stable, so the
@@ -534,12 +533,12 @@ __global__ __launch_bounds__(NT, 2 * NT / 256) void kernel_radix_onesweep(
 
 // Which digit-pass structure radix_sort uses (a process-wide setting):
 // 0 = automatic, 1 = count + scan + scatter, 2 = single pass where eligible
-// (count < 2^30).  Automatic takes the single pass up to 2^22 keys, where its
-// fewer launches (3 + passes instead of 4 per pass) make it faster, and count
-// + scan + scatter above: measured on MI355X with 2^28 uint32 pairs, the
-// single pass's look-back over ~26k tiles costs 1.42 ms per pass against
-// 0.88 + 0.19 ms for scatter + count.
-constexpr int64_t radix_single_pass_auto_max = (int64_t)1 << 22;
+// (count < 2^30).  Automatic is count + scan + scatter at every size: on
+// MI355X the single pass's look-back costs 1.42 ms per pass at 2^28 uint32
+// pairs against 0.88 + 0.19 ms for scatter + count, and small sorts are
+// faster with count + scan + scatter over small tiles (1M keys, 512 x 8
+// tiles: 0.068 ms) than with the single pass (0.077 ms at 512 x 8, 0.079 ms
+// at 512 x 20).
 inline std::atomic<int>& radix_algorithm_setting() {
   static std::atomic<int> a{0};
   return a;
@@ -596,8 +595,7 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
   uint32_t* v_out = vals2;
   hipError_t e;
   const int algo = radix_algorithm_setting().load(std::memory_order_relaxed);
-  const bool single_pass = algo == 2 ? count < ((int64_t)1 << 30)
-                                     : algo == 0 && count <= radix_single_pass_auto_max;
+  const bool single_pass = algo == 2 && count < ((int64_t)1 << 30);
   if (single_pass) {
     uint32_t* status = advance_pointer<uint32_t>(p, num_counts * D);
     uint32_t* all_counts = advance_pointer<uint32_t>(p, (int64_t)D * radix_num_bins);

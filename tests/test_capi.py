@@ -71,19 +71,36 @@ def test_sizing_accumulates():
     assert size.value == 1000 + r256(8 * (-(-10000 // (512 * 23)) + 1))  # merge tile
 
 
-def test_sizing_radix_and_scan():
-    lib = mgpu_hip.load()
-    n = 1 << 20
-    st, sz = _size(lib.mgpu_radix_sort_kv_u32_u32, ctypes.c_void_p(1), ctypes.c_void_p(1), n)
-    tiles = -(-n // (512 * 20))
-    assert st == 0
+def radix_aux(n, key_bytes, val_bytes, nv, passes):
+    """Ping-pong buffers + max(count + scan path, single-pass path)."""
+    tiles = -(-n // nv)
     scan_aux = r256(8 * (-(-tiles * 256 // 4096)))  # counts scan: 4096-element tiles
     scan_path = r256(4 * tiles * 256) + scan_aux
-    # Single pass: [4 passes][tiles][256] status words, all-pass counts, bases.
-    single_pass = r256(4 * tiles * 256 * 4) + 2 * r256(4 * 4 * 256)
-    assert sz == r256(4 * n) * 2 + max(scan_path, single_pass)
-    st, sz = _size(lib.mgpu_scan_exclusive_i32, ctypes.c_void_p(1), n, 0)
-    assert st == 0 and sz == r256(8 * -(-n // (256 * 80)))  # 8 B per 20480-element tile
+    # Single pass: [passes][tiles][256] status words, all-pass counts, bases.
+    single_pass = r256(4 * tiles * 256 * passes) + 2 * r256(4 * passes * 256)
+    return r256(key_bytes * n) + (r256(val_bytes * n) if val_bytes else 0) + \
+        max(scan_path, single_pass)
+
+
+def test_sizing_radix_and_scan():
+    lib = mgpu_hip.load()
+    one = ctypes.c_void_p(1)
+    # Tile shape by size: 32-bit kv 512 x 8 up to 6 * 2^20 keys, 512 x 20 above;
+    # 32-bit keys 512 x 8 / 512 x 12 / 512 x 24 (cuts at 2^22, 2^25); 64-bit
+    # keys 512 x 8 up to 2^22, 512 x 12 above.
+    for fn, args, n, kb, vb, nv, passes in [
+            (lib.mgpu_radix_sort_kv_u32_u32, (one, one), 1 << 20, 4, 4, 4096, 4),
+            (lib.mgpu_radix_sort_kv_u32_u32, (one, one), 7 << 20, 4, 4, 10240, 4),
+            (lib.mgpu_radix_sort_keys_f32, (one,), 1 << 22, 4, 0, 4096, 4),
+            (lib.mgpu_radix_sort_keys_f32, (one,), (1 << 22) + 1, 4, 0, 6144, 4),
+            (lib.mgpu_radix_sort_keys_u32, (one,), (1 << 25) + 1, 4, 0, 12288, 4),
+            (lib.mgpu_radix_sort_kv_u64_u32, (one, one), 1 << 22, 8, 4, 4096, 8),
+            (lib.mgpu_radix_sort_kv_u64_u32, (one, one), (1 << 22) + 1, 8, 4, 6144, 8)]:
+        st, sz = _size(fn, *args, n)
+        assert st == 0
+        assert sz == radix_aux(n, kb, vb, nv, passes), (fn, n)
+    st, sz = _size(lib.mgpu_scan_exclusive_i32, one, 1 << 20, 0)
+    assert st == 0 and sz == r256(8 * -(-(1 << 20) // (256 * 80)))  # 8 B per 20480-element tile
 
 
 def test_invalid_arguments():

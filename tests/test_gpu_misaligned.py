@@ -34,7 +34,7 @@ def check_guards(buf, off, n, sentinel):
 @pytest.mark.parametrize("off", OFFSETS)
 def test_radix_kv_misaligned(off):
     banner("radix misaligned", off)
-    n = 3 * 512 * 20 + 77
+    n = 3 * 512 * 8 + 77
     keys = oracle.generate(n, "u32", 11)[np.arange(n) % 4099]
     vals = np.arange(n, dtype=np.uint32)
     kb, kv, ks = padded(keys, off)

@@ -13,7 +13,7 @@ KTS = ("u32", "i32", "u64", "f32", "i64", "f64")
 MS_NV = 512 * 19      # mergesort tile of 32-bit keys in the C ABI instantiation
 MS_NV64 = 256 * 15    # mergesort and merge tile of 64-bit keys
 MERGE_NV = 512 * 23   # merge tile of 32-bit keys
-RADIX_NV = 512 * 20   # radix tile (u32/i32/f32)
+RADIX_NV = 512 * 8    # radix tile of every size below 2^22 keys
 SIZES = [1, 2, 7, 64, 100, MS_NV64 - 1, MS_NV64, MS_NV64 + 1, MS_NV - 1, MS_NV, MS_NV + 1,
          2 * MS_NV + 3, 5 * MS_NV - 11, RADIX_NV + 1, 100003, (1 << 20) + 17]
 
@@ -252,6 +252,31 @@ def test_radix_repeat_is_deterministic_and_idempotent(radix_algo):
     k3, v3 = k1.clone(), v1.clone()
     mgpu_hip.radix_sort(k3, v3)
     assert dev_equal(k3, k1) and dev_equal(v3, v1)
+
+
+@pytest.mark.parametrize("kt,kv,n", [
+    ("u32", True, 6 * (1 << 20)),             # last count on 512 x 8 tiles (kv)
+    ("u32", True, 615 * 10240 + 1),           # 512 x 20 tiles, one key past a tile edge
+    ("f32", False, 1 << 22),                  # last count on 512 x 8 tiles (keys)
+    ("i32", False, 683 * 6144),               # 512 x 12 tiles, exactly whole tiles
+    ("u32", False, 2731 * 12288 + 5),         # 512 x 24 tiles
+    ("u64", True, 683 * 6144 + 3),            # 64-bit keys on 512 x 12 tiles
+])
+def test_radix_tile_shape_boundaries(kt, kv, n, radix_algo):
+    """Sizes on either side of the counts where the library changes tile shape."""
+    banner("radix shape boundary", kt, kv, n, radix_algo)
+    keys = make_keys(kt, n, 21, "ties")
+    dk = dev_keys(keys)
+    if kv:
+        vals = np.arange(n, dtype=np.uint32)
+        dv = to_dev(vals)
+        mgpu_hip.radix_sort(dk, dv)
+        ek, ev = oracle.stable_sort_radix(keys, vals, kt)
+        np.testing.assert_array_equal(to_host(dv), ev)
+    else:
+        mgpu_hip.radix_sort(dk)
+        ek, _ = oracle.stable_sort_radix(keys, np.zeros(n, np.uint32), kt)
+    np.testing.assert_array_equal(to_host(dk).view(np.uint8), ek.view(np.uint8))
 
 
 # ------------------------------------------------------ look-back fallback path
