@@ -99,11 +99,17 @@ __device__ inline void cta_radix_count(const key_t* keys, int tile_count, int sh
 }
 
 // Per-tile counts of one digit, digit-major: counts[d * num_tiles + tile].
+// The grid also zeroes clear[0, clear_words) (the status words of the scan
+// that follows), which saves that scan its own clearing launch.
 template <int CNT, int NV, typename key_t>
 __global__ __launch_bounds__(CNT) void kernel_radix_count(const key_t* keys, int64_t count,
                                                           int shift, uint32_t* counts,
-                                                          int64_t num_tiles) {
+                                                          int64_t num_tiles, uint32_t* clear,
+                                                          int64_t clear_words) {
   __shared__ uint32_t h[radix_num_bins * 16];
+  for (int64_t i = (int64_t)blockIdx.x * CNT + threadIdx.x; i < clear_words;
+       i += (int64_t)gridDim.x * CNT)
+    clear[i] = 0;
   const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
   const int64_t base = tile * NV;
   const int tile_count = (int)min_<int64_t>(NV, count - base);
@@ -624,16 +630,19 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
     return hip_status(e);
   }
   uint32_t* counts = advance_pointer<uint32_t>(p, num_counts);
-  size_t scan_cap = aux_size - ((char*)p - (char*)aux_data);
+  // The counts scan's status words (one 64-bit word per 4096 counts, at the
+  // start of its aux) are zeroed by the count kernel of each pass.
+  uint64_t* scan_status = (uint64_t*)p;
+  const int64_t scan_words = 2 * div_up(num_counts, (int64_t)(256 * 16));
   for (int pass = 0; pass < D; ++pass) {
     const int shift = 8 * pass;
     e = launch_kernel("mgpu_radix_count", kernel_radix_count<CNT, NV, key_t>,
                       dim3((unsigned)num_tiles), dim3(CNT), 0, stream, (const key_t*)k_in, count,
-                      shift, counts, num_tiles);
+                      shift, counts, num_tiles, (uint32_t*)scan_status, scan_words);
     if (e != hipSuccess) return hip_status(e);
-    mgpu_status_t st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)counts, counts,
-                                     num_counts, MGPU_SCAN_EXCLUSIVE, 0u);
-    if (st != MGPU_OK) return st;
+    e = scan_launch_cleared<256, 16>(stream, (const uint32_t*)counts, counts, num_counts,
+                                     MGPU_SCAN_EXCLUSIVE, 0u, identity_t(), scan_status);
+    if (e != hipSuccess) return hip_status(e);
     const int64_t num_full = count / NV;
     e = launch_kernel("mgpu_radix_scatter", kernel_radix_scatter<NT, VT, key_t, has_values>,
                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, (const key_t*)k_in, k_out,

@@ -270,6 +270,25 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
   }
 }
 
+// The scan kernel of transform_scan, for a caller that has already zeroed the
+// status words (one 64-bit word per NT * VT tile; transform_scan keeps them at
+// the start of its aux).  Arguments are not checked again.
+template <int NT, int VT, typename in_t, typename T, typename xform_t>
+hipError_t scan_launch_cleared(hipStream_t stream, const in_t* input, T* output, int64_t count,
+                               mgpu_scan_type_t type, T init, xform_t xform, uint64_t* status) {
+  constexpr int NV = NT * VT;
+  const int64_t num_tiles = div_up(count, (int64_t)NV);
+  const lookback_config_t lb = lookback_config();
+  int vec_ok = sizeof(in_t) == 4 && 0 == ((uintptr_t)input & 15) && 0 == ((uintptr_t)output & 15);
+  if (type == MGPU_SCAN_INCLUSIVE)
+    return launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, true, in_t, T, xform_t>,
+                         dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
+                         num_tiles, init, status, lb, vec_ok, xform);
+  return launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, false, in_t, T, xform_t>,
+                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
+                       num_tiles, init, status, lb, vec_ok, xform);
+}
+
 // transform_scan: scan of xform(input[i]).  input/output may alias (in place).
 template <int NT = 256, int VT = 80, typename in_t, typename T, typename xform_t>
 mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
@@ -290,19 +309,10 @@ mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t strea
   if (aux_size < need) return MGPU_AUX_TOO_SMALL;
   void* p = aux_data;
   uint64_t* status = advance_pointer<uint64_t>(p, num_tiles);
-  const lookback_config_t lb = lookback_config();
   hipError_t e = fill_u32((uint32_t*)status, 2 * num_tiles, 0u, stream);
   if (e != hipSuccess) return hip_status(e);
-  int vec_ok = sizeof(in_t) == 4 && 0 == ((uintptr_t)input & 15) && 0 == ((uintptr_t)output & 15);
-  if (type == MGPU_SCAN_INCLUSIVE)
-    e = launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, true, in_t, T, xform_t>,
-                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
-                      num_tiles, init, status, lb, vec_ok, xform);
-  else
-    e = launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, false, in_t, T, xform_t>,
-                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
-                      num_tiles, init, status, lb, vec_ok, xform);
-  return hip_status(e);
+  return hip_status(scan_launch_cleared<NT, VT>(stream, input, output, count, type, init, xform,
+                                                status));
 }
 
 template <int NT = 256, int VT = 80, typename T>
