@@ -25,12 +25,13 @@ def make_keys(kt, n, seed, ties=True):
 
 
 def boundary_lengths():
-    """Lengths on both sides of every bucket capacity and the radix tile."""
-    caps = [256, 512, 1024, 2048, 4096, 8192, 12288]
+    """Lengths on both sides of every bucket capacity and blocksort shape
+    capacity, and of one, two and three large-segment radix tiles (8192)."""
+    caps = [256, 320, 512, 576, 1024, 1152, 2048, 2304, 4096, 4352, 8192, 8704, 12288, 16384]
     lens = [0, 1, 2, 3, 0, 1]
     for c in caps:
         lens += [c - 1, c, c + 1]
-    lens += [2 * 12288 + 5, 3 * 12288, 70000]
+    lens += [2 * 8192 + 5, 3 * 8192, 3 * 12288, 70000]
     return np.array(lens, dtype=np.int64)
 
 
