@@ -64,8 +64,9 @@ extern "C" mgpu_status_t mgpu_reduce_u64(void* aux, size_t* aux_size, mgpu_strea
       int64_t count, const int32_t* segments, int64_t num_segments, mgpu_scan_type_t type,  \
       TT init) {                                                                            \
     if (!aux_size) return MGPU_INVALID_ARG;                                                 \
-    return segmented_scan<256, 32>(aux, *aux_size, (hipStream_t)stream, input, output,      \
-                                   count, segments, num_segments, type, init);              \
+    return segmented_scan<segscan_tuning<TT>::nt, segscan_tuning<TT>::vt>(                   \
+        aux, *aux_size, (hipStream_t)stream, input, output, count, segments, num_segments,   \
+        type, init);                                                                        \
   }
 
 MGPU_DEF_SEGSCAN(i32, int32_t)

@@ -1,6 +1,7 @@
 // Launch shapes used by the C ABI instantiations.
 #pragma once
 #include <cstdint>
+#include <type_traits>
 
 // Merge / mergesort tiles, measured on MI355X with 32-bit keys: merge of
 // 2 x 2^27 keys 0.404 ms at 512 x 23 (256 x 15: 0.489); mergesort of 2^28
@@ -14,3 +15,10 @@ template <typename key_t> struct mergesort_tuning<key_t, 8> { static constexpr i
 // steps per byte; 2^29 int32 exclusive: 0.775 ms at 256 x 80 vs 1.19 ms at
 // 256 x 16 on MI355X (float32 inclusive 0.822 vs 1.26 ms).
 struct scan_tuning { static constexpr int nt = 256, vt = 80; };
+// Segmented scan: integers 256 x 48, float32 256 x 32.  2^29 int32 over
+// 65536 segments, exclusive: 0.843 ms at 256 x 48, 0.890 at 256 x 32, 0.854
+// at 256 x 64 (inclusive 0.889 / 0.921 / 1.022).  The float32 kernels carry
+// float64 partial sums and at 256 x 48 would need 244 VGPRs (2 waves/SIMD).
+template <typename T> struct segscan_tuning {
+  static constexpr int nt = 256, vt = std::is_floating_point<T>::value ? 32 : 48;
+};

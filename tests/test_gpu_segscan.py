@@ -14,13 +14,14 @@ from mgpu_hip import workloads
 
 pytestmark = pytest.mark.gpu
 
-NV = 256 * 32  # elements per tile of the library's segmented scan
+NV_INT = 256 * 48  # elements per tile of the library's segmented scan: integers
+NV_F32 = 256 * 32  # float32
 F32_SCAN_RTOL = 2e-6
-SIZES = [1, 5, 4095, 4096, 4097, 3 * NV + 7, 1 << 20, (1 << 22) + 13]
+SIZES = [1, 5, 4095, 4096, 4097, 3 * NV_INT + 7, 3 * NV_F32 + 7, 1 << 20, (1 << 22) + 13]
 LAYOUTS = ["none", "single0", "every", "lognormal", "tile_edges", "sparse", "dupes"]
 
 
-def layout(name, n, seed=0):
+def layout(name, n, seed=0, nv=NV_INT):
     if name == "none":
         return np.zeros(0, np.int32)
     if name == "single0":
@@ -32,13 +33,13 @@ def layout(name, n, seed=0):
         s = workloads.segment_starts(lens)
         return s[s < n]
     if name == "tile_edges":
-        t = np.arange(0, n + NV, NV, dtype=np.int64)
+        t = np.arange(0, n + nv, nv, dtype=np.int64)
         s = np.unique(np.concatenate([t - 1, t, t + 1]))
         return s[(s >= 0) & (s < n)].astype(np.int32)
     if name == "sparse":  # few starts, segments over many tiles
         return np.array([0, n // 3, n // 3 + 1, (2 * n) // 3], dtype=np.int32)
     if name == "dupes":  # empty segments, starts past the end
-        s = np.array([0, 7, 7, 7, NV, NV, n // 2, n, n, n + 9], dtype=np.int64)
+        s = np.array([0, 7, 7, 7, nv, nv, n // 2, n, n, n + 9], dtype=np.int64)
         return np.sort(np.clip(s, 0, n + 9)).astype(np.int32)
     raise ValueError(name)
 
@@ -75,12 +76,12 @@ def test_segscan_u32_inclusive_wraps(n, lay):
     check("u32", n, layout(lay, n, 2), True, 0xFFFFFFF0)
 
 
-@pytest.mark.parametrize("n", [4095, 3 * NV + 7, (1 << 22) + 13])
+@pytest.mark.parametrize("n", [4095, 3 * NV_F32 + 7, (1 << 22) + 13])
 @pytest.mark.parametrize("lay", LAYOUTS)
 @pytest.mark.parametrize("inclusive", [False, True])
 def test_segscan_f32_error_bound(n, lay, inclusive):
     banner("segscan f32", n, lay, inclusive)
-    check("f32_01", n, layout(lay, n, 3), inclusive, 0.0)
+    check("f32_01", n, layout(lay, n, 3, NV_F32), inclusive, 0.0)
 
 
 @pytest.mark.parametrize("lay", ["lognormal", "sparse", "tile_edges"])
@@ -88,7 +89,7 @@ def test_segscan_in_place(lay):
     banner("segscan in place", lay)
     n = (1 << 21) + 77
     check("i32_0_16", n, layout(lay, n, 4), False, 0, in_place=True)
-    check("f32_01", n, layout(lay, n, 4), True, 0.0, in_place=True)
+    check("f32_01", n, layout(lay, n, 4, NV_F32), True, 0.0, in_place=True)
 
 
 @pytest.mark.parametrize("in_place", [False, True])
@@ -102,7 +103,7 @@ def test_segscan_forced_lookback_fallback(in_place):
         n = (1 << 22) + 13
         for lay in ("sparse", "lognormal", "none"):
             check("i32_0_16", n, layout(lay, n, 6), False, 1, in_place=in_place)
-            check("f32_01", n, layout(lay, n, 6), True, 0.0, in_place=in_place)
+            check("f32_01", n, layout(lay, n, 6, NV_F32), True, 0.0, in_place=in_place)
     finally:
         mgpu_hip.set_lookback_spin_limit(old)
 
