@@ -117,6 +117,29 @@ __global__ __launch_bounds__(CNT) void kernel_radix_count(const key_t* keys, int
                            counts + tile, num_tiles, h);
 }
 
+// Per-tile counts of digit bytes written by the previous pass's scatter
+// (digits[i] = digit of the key at position i), digit-major like
+// kernel_radix_count; also zeroes clear[0, clear_words).
+struct radix_byte_bits_t {
+  __device__ uint32_t operator()(uint8_t b) const { return b; }
+};
+template <int NV>
+__global__ __launch_bounds__(128) void kernel_radix_count_digits(const uint8_t* digits,
+                                                                 int64_t count, uint32_t* counts,
+                                                                 int64_t num_tiles,
+                                                                 uint32_t* clear,
+                                                                 int64_t clear_words) {
+  __shared__ uint32_t h[radix_num_bins * 16];
+  for (int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x; i < clear_words;
+       i += (int64_t)gridDim.x * 128)
+    clear[i] = 0;
+  const int64_t tile = xcd_tile(blockIdx.x, num_tiles);
+  const int64_t base = tile * NV;
+  const int tile_count = (int)min_<int64_t>(NV, count - base);
+  cta_radix_count<128, NV>(digits + base, tile_count, 0, radix_byte_bits_t(), counts + tile,
+                           num_tiles, h);
+}
+
 template <int NT, int VT, typename key_t>
 struct radix_scatter_storage_t {
   static constexpr int NW = NT / warp_size;
@@ -154,12 +177,13 @@ struct radix_fixed_offsets_t {
   __device__ uint32_t resolve(int, uint32_t) const { return goff; }
 };
 
-template <int NT, int VT, typename key_t, bool has_values, bool full = false, typename xbits_t,
-          typename offs_t>
+template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
+          typename xbits_t, typename offs_t>
 __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
                                          key_t* keys_out, uint32_t* vals_out, int tile_count,
                                          int shift, xbits_t xb, offs_t& offs, int64_t out_limit,
-                                         radix_scatter_storage_t<NT, VT, key_t>& s) {
+                                         radix_scatter_storage_t<NT, VT, key_t>& s,
+                                         uint8_t* digits_out = nullptr, int next_shift = 0) {
   constexpr int NW = NT / warp_size;
   static_assert(NT >= radix_num_bins && NT % radix_num_bins == 0, "NT must cover the bins");
   static_assert(sizeof(key_t) * NT * VT >= sizeof(uint64_t) * NW * radix_num_bins,
@@ -303,7 +327,10 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
     const key_t key = s.keys[p];
     const uint32_t d = (uint32_t)(xb(key) >> shift) & 255u;
     dst[i] = s.global_off[d] + p;
-    if ((full || p <= last) && dst[i] < out_limit) keys_out[dst[i]] = key;
+    if ((full || p <= last) && dst[i] < out_limit) {
+      keys_out[dst[i]] = key;
+      if constexpr (emit) digits_out[dst[i]] = (uint8_t)((uint32_t)(xb(key) >> next_shift) & 255u);
+    }
   }
   if constexpr (has_values) {
     __syncthreads();
@@ -319,25 +346,31 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   }
 }
 
-template <int NT, int VT, typename key_t, bool has_values, bool full = false, typename xbits_t>
+template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
+          typename xbits_t>
 __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
                                          key_t* keys_out, uint32_t* vals_out, int tile_count,
                                          int shift, xbits_t xb, uint32_t goff, int64_t out_limit,
-                                         radix_scatter_storage_t<NT, VT, key_t>& s) {
+                                         radix_scatter_storage_t<NT, VT, key_t>& s,
+                                         uint8_t* digits_out = nullptr, int next_shift = 0) {
   radix_fixed_offsets_t offs{goff};
-  cta_radix_scatter<NT, VT, key_t, has_values, full>(keys_in, vals_in, keys_out, vals_out,
-                                                     tile_count, shift, xb, offs, out_limit, s);
+  cta_radix_scatter<NT, VT, key_t, has_values, full, emit>(keys_in, vals_in, keys_out, vals_out,
+                                                           tile_count, shift, xb, offs,
+                                                           out_limit, s, digits_out, next_shift);
 }
 
 // Scatter of one digit pass.  offsets[d * num_tiles + t] is the global
 // position of tile t's first key with digit d.  Blocks [0, num_full) take
 // the full tiles (dealt XCD-contiguously); a block num_full, if launched,
-// takes the trailing partial tile.
+// takes the trailing partial tile.  With emit, each key's digit of the next
+// pass is also written as one byte at its output position, so that the next
+// pass counts 1 byte per key instead of reading the keys.
 // Two blocks per CU (the LDS allows two): at most 128 VGPRs.
-template <int NT, int VT, typename key_t, bool has_values>
+template <int NT, int VT, typename key_t, bool has_values, bool emit>
 __global__ __launch_bounds__(NT, 2 * NT / 256) void kernel_radix_scatter(
     const key_t* keys_in, key_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
-    int64_t count, int shift, const uint32_t* offsets, int64_t num_tiles, int64_t num_full) {
+    int64_t count, int shift, const uint32_t* offsets, int64_t num_tiles, int64_t num_full,
+    uint8_t* digits_out) {
   constexpr int NV = NT * VT;
   __shared__ radix_scatter_storage_t<NT, VT, key_t> s;
   const int tid = threadIdx.x;
@@ -350,13 +383,13 @@ __global__ __launch_bounds__(NT, 2 * NT / 256) void kernel_radix_scatter(
   const key_t* kin = keys_in + base;
   const uint32_t* vin = has_values ? vals_in + base : vals_in;
   if (full)
-    cta_radix_scatter<NT, VT, key_t, has_values, true>(kin, vin, keys_out, vals_out, tile_count,
-                                                       shift, radix_bits_t<key_t>(), goff,
-                                                       count, s);
+    cta_radix_scatter<NT, VT, key_t, has_values, true, emit>(
+        kin, vin, keys_out, vals_out, tile_count, shift, radix_bits_t<key_t>(), goff, count, s,
+        digits_out, shift + radix_num_bits);
   else
-    cta_radix_scatter<NT, VT, key_t, has_values, false>(kin, vin, keys_out, vals_out,
-                                                        tile_count, shift, radix_bits_t<key_t>(),
-                                                        goff, count, s);
+    cta_radix_scatter<NT, VT, key_t, has_values, false, emit>(
+        kin, vin, keys_out, vals_out, tile_count, shift, radix_bits_t<key_t>(), goff, count, s,
+        digits_out, shift + radix_num_bits);
 }
 
 ////////////////////////////////////////////////////////////////////////////////
@@ -550,6 +583,14 @@ inline std::atomic<int>& radix_algorithm_setting() {
   return a;
 }
 
+// Whether the count + scan + scatter passes carry each key's next digit as a
+// byte (written by the scatter, read by the next count): for 64-bit keys the
+// count then reads 1 byte per key instead of 8 (2^27 u64 pairs: 7.13 ->
+// 6.78 ms).  For 32-bit keys the scattered byte stores cost the scatter more
+// than the count saves (2^28 u32 pairs: 4.35 -> 4.71 ms).
+template <typename key_t>
+constexpr bool radix_digit_bytes() { return sizeof(key_t) == 8; }
+
 template <int NT, int VT, typename key_t, typename val_t>
 inline size_t radix_aux_bytes(int64_t count) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
@@ -564,7 +605,8 @@ inline size_t radix_aux_bytes(int64_t count) {
   // The single-pass path instead needs [passes][tiles][256] status words and
   // the all-pass counts and bases; size for both so that the choice can be
   // made when the sort is enqueued.
-  const size_t scan_path = aux_bytes<uint32_t>(num_counts) + scan_need;
+  const size_t scan_path = aux_bytes<uint32_t>(num_counts) + scan_need +
+                           (radix_digit_bytes<key_t>() ? aux_bytes<uint8_t>(count) : 0);
   const size_t one_path = aux_bytes<uint32_t>(num_counts * (int64_t)sizeof(key_t)) +
                           2 * aux_bytes<uint32_t>((int64_t)sizeof(key_t) * radix_num_bins);
   return need - aux_bytes<uint32_t>(num_counts) + max_(scan_path, one_path);
@@ -629,25 +671,40 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
     }
     return hip_status(e);
   }
+  constexpr bool emit = radix_digit_bytes<key_t>();
+  uint8_t* digits = emit ? advance_pointer<uint8_t>(p, count) : nullptr;
   uint32_t* counts = advance_pointer<uint32_t>(p, num_counts);
   // The counts scan's status words (one 64-bit word per 4096 counts, at the
   // start of its aux) are zeroed by the count kernel of each pass.
   uint64_t* scan_status = (uint64_t*)p;
   const int64_t scan_words = 2 * div_up(num_counts, (int64_t)(256 * 16));
+  const int64_t num_full = count / NV;
   for (int pass = 0; pass < D; ++pass) {
     const int shift = 8 * pass;
-    e = launch_kernel("mgpu_radix_count", kernel_radix_count<CNT, NV, key_t>,
-                      dim3((unsigned)num_tiles), dim3(CNT), 0, stream, (const key_t*)k_in, count,
-                      shift, counts, num_tiles, (uint32_t*)scan_status, scan_words);
+    // Pass 0 counts digits from the keys; with digit bytes, later passes
+    // count the bytes the previous scatter wrote.
+    if (pass == 0 || !emit)
+      e = launch_kernel("mgpu_radix_count", kernel_radix_count<CNT, NV, key_t>,
+                        dim3((unsigned)num_tiles), dim3(CNT), 0, stream, (const key_t*)k_in,
+                        count, shift, counts, num_tiles, (uint32_t*)scan_status, scan_words);
+    else
+      e = launch_kernel("mgpu_radix_count_digits", kernel_radix_count_digits<NV>,
+                        dim3((unsigned)num_tiles), dim3(128), 0, stream, (const uint8_t*)digits,
+                        count, counts, num_tiles, (uint32_t*)scan_status, scan_words);
     if (e != hipSuccess) return hip_status(e);
     e = scan_launch_cleared<256, 16>(stream, (const uint32_t*)counts, counts, num_counts,
                                      MGPU_SCAN_EXCLUSIVE, 0u, identity_t(), scan_status);
     if (e != hipSuccess) return hip_status(e);
-    const int64_t num_full = count / NV;
-    e = launch_kernel("mgpu_radix_scatter", kernel_radix_scatter<NT, VT, key_t, has_values>,
-                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, (const key_t*)k_in, k_out,
-                      (const uint32_t*)v_in, v_out, count, shift, (const uint32_t*)counts,
-                      num_tiles, num_full);
+    if (emit && pass + 1 < D)
+      e = launch_kernel("mgpu_radix_scatter", kernel_radix_scatter<NT, VT, key_t, has_values, emit>,
+                        dim3((unsigned)num_tiles), dim3(NT), 0, stream, (const key_t*)k_in, k_out,
+                        (const uint32_t*)v_in, v_out, count, shift, (const uint32_t*)counts,
+                        num_tiles, num_full, digits);
+    else
+      e = launch_kernel("mgpu_radix_scatter", kernel_radix_scatter<NT, VT, key_t, has_values, false>,
+                        dim3((unsigned)num_tiles), dim3(NT), 0, stream, (const key_t*)k_in, k_out,
+                        (const uint32_t*)v_in, v_out, count, shift, (const uint32_t*)counts,
+                        num_tiles, num_full, (uint8_t*)nullptr);
     if (e != hipSuccess) return hip_status(e);
     std::swap(k_in, k_out);
     std::swap(v_in, v_out);

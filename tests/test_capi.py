@@ -75,7 +75,9 @@ def radix_aux(n, key_bytes, val_bytes, nv, passes):
     """Ping-pong buffers + max(count + scan path, single-pass path)."""
     tiles = -(-n // nv)
     scan_aux = r256(8 * (-(-tiles * 256 // 4096)))  # counts scan: 4096-element tiles
-    scan_path = r256(4 * tiles * 256) + scan_aux
+    # Count path: per-tile counts, their scan, and for 64-bit keys one digit
+    # byte per key.
+    scan_path = r256(4 * tiles * 256) + scan_aux + (r256(n) if key_bytes == 8 else 0)
     # Single pass: [passes][tiles][256] status words, all-pass counts, bases.
     single_pass = r256(4 * tiles * 256 * passes) + 2 * r256(4 * passes * 256)
     return r256(key_bytes * n) + (r256(val_bytes * n) if val_bytes else 0) + \
