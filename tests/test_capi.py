@@ -37,6 +37,25 @@ def test_status_strings():
     assert lib.mgpu_version() >= 100
 
 
+def test_radix_algorithm_setting():
+    """0 automatic, 1 count + scan + scatter, 2 single pass; anything else
+    falls back to automatic.  Sizing does not depend on the setting."""
+    old = mgpu_hip.radix_algorithm()
+    one = ctypes.c_void_p(1)
+    try:
+        sizes = []
+        for a in (0, 1, 2):
+            mgpu_hip.set_radix_algorithm(a)
+            assert mgpu_hip.radix_algorithm() == a
+            sizes.append(_size(mgpu_hip.load().mgpu_radix_sort_kv_u32_u32, one, one, 1 << 20))
+        assert sizes[0] == sizes[1] == sizes[2]
+        for bad in (-1, 3, 99):
+            mgpu_hip.set_radix_algorithm(bad)
+            assert mgpu_hip.radix_algorithm() == 0
+    finally:
+        mgpu_hip.set_radix_algorithm(old)
+
+
 def _size(fn, *args):
     size = ctypes.c_size_t(0)
     st = fn(None, ctypes.byref(size), None, *args)
