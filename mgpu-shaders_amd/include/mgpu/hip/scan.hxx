@@ -436,138 +436,237 @@ mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const
 }
 
 ////////////////////////////////////////////////////////////////////////////////
-// 64-bit integer scan (modulo 2^64).  A 64-bit sum and a status flag do not
-// fit one atomic word, so this scan is three phases in the reference's own
-// structure (vk/scan.hxx:65-113): tile reductions, an exclusive scan of the
-// tile partials (recursively, by the same three phases), then a downsweep
-// that rescans each tile with its carry.  12 bytes moved per element.
+// 64-bit integer scan (modulo 2^64), single pass with decoupled look-back like
+// the 32-bit scan.  A 64-bit value and a flag do not fit one atomic word, so a
+// tile's status is two 64-bit words, each carrying the flag in its high half
+// and one 32-bit half of the value in its low half.  Each word is written and
+// read atomically; a reader takes the status only when both words carry the
+// same flag (a torn pair, one word updated and the other not yet, is re-read),
+// so no ordering between the two stores is needed.  8 bytes read and 8
+// written per element.
 
-template <int NT, int VT, typename T>
-__global__ __launch_bounds__(NT) void kernel_scan64_reduce(const T* input, int64_t count,
-                                                           T* partials) {
-  constexpr int NV = NT * VT;
-  __shared__ T s_warp[NT / warp_size];
-  const int64_t base = (int64_t)blockIdx.x * NV;
-  T sum = 0;
-#pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    int64_t idx = base + i * NT + threadIdx.x;
-    if (idx < count) sum += input[idx];
-  }
-  T total = block_reduce<NT>(sum, s_warp);
-  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+__device__ inline void scan64_publish(uint64_t* status, int64_t tile, uint64_t v, uint32_t flag) {
+  const uint64_t hi_flag = (uint64_t)flag << 32;
+  atomic_store_agent(status + 2 * tile, hi_flag | (v & (uint64_t)0xFFFFFFFFu));
+  atomic_store_agent(status + 2 * tile + 1, hi_flag | (v >> 32));
 }
 
-// Exclusive block scan of NT*VT elements per tile with a carry per tile;
-// carries == nullptr means carry 0 (single tile).  Thread-serial over VT
-// consecutive elements, then a wave scan, then the wave totals.
+// Flag of a word pair: the common flag, or 3 for a torn pair.
+constexpr uint32_t status64_torn = 3;
+__device__ inline uint32_t scan64_read(const uint64_t* status, int64_t tile, uint64_t& v) {
+  const uint64_t w0 = atomic_load_agent(status + 2 * tile);
+  const uint64_t w1 = atomic_load_agent(status + 2 * tile + 1);
+  v = (w1 << 32) | (w0 & (uint64_t)0xFFFFFFFFu);
+  const uint32_t f0 = (uint32_t)(w0 >> 32), f1 = (uint32_t)(w1 >> 32);
+  return f0 == f1 ? f0 : status64_torn;
+}
+
+// Exclusive prefix of tile `tile` (one full wave; every lane returns it), as
+// scan_lookback: 64 predecessors per round, bounded spin, then the first
+// missing tile's sum from its input with the in-place re-check.  A torn pair
+// means its owner is between its two stores, so it is only ever re-polled.
+template <int NV, typename T>
+__device__ inline uint64_t scan64_lookback(const uint64_t* status, int64_t tile, uint64_t init,
+                                           const T* input, lookback_config_t lb) {
+  const int lane = lane_id();
+  uint64_t excl = 0;
+  int64_t j = tile - 1;
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t idx = j - lane;
+    uint64_t v = 0;
+    uint32_t flag;
+    if (idx >= 0) {
+      flag = scan64_read(status, idx, v);
+    } else {
+      flag = status_inclusive;
+      v = idx == -1 ? init : 0;
+    }
+    const uint64_t inc_mask = __ballot(flag == status_inclusive);
+    const uint64_t bad_mask = __ballot(flag == status_invalid || flag == status64_torn);
+    const uint64_t lowest = inc_mask & (~inc_mask + 1);
+    const uint64_t range = inc_mask ? (lowest | (lowest - 1)) : ~0ull;
+    if (bad_mask & range) {
+      const int kb = __builtin_ctzll(bad_mask & range);
+      const bool torn = __shfl(flag, kb, warp_size) == status64_torn;
+      if (++spins <= lb.spin_limit || torn) {
+        __builtin_amdgcn_s_sleep(2);
+        continue;
+      }
+      const int64_t jj = j - kb;
+      uint64_t agg = 0;
+      for (int e = lane; e < NV; e += warp_size) agg += (uint64_t)input[jj * NV + e];
+      agg = wave_reduce_all(agg);
+      uint64_t again;
+      const uint32_t f_again = scan64_read(status, jj, again);
+      excl += wave_reduce_all(lane < kb ? v : (uint64_t)0);
+      spins = 0;
+      if (f_again != status_invalid) {
+        j = jj;  // published meanwhile: continue from its words
+        continue;
+      }
+      excl += agg;
+      if (lane == 0 && lb.fallbacks) atomic_add_agent(lb.fallbacks, 1u);
+      j = jj - 1;
+      continue;
+    }
+    excl += wave_reduce_all(((range >> lane) & 1) ? v : (uint64_t)0);
+    if (inc_mask) break;
+    j -= warp_size;
+  }
+  return excl;
+}
+
+// One tile of NT * VT elements in VT / 2 rows of 2 * NT: each thread loads two
+// consecutive elements of every row (one 16-byte load), sums them, and a wave
+// scan of the pair sums plus the row's wave totals (in LDS) give every prefix
+// inside the tile; the tile's carry comes from the look-back.  Large tiles
+// keep the look-back off the critical path (the tiles' status words are
+// exchanged across XCDs at a few memory round trips per tile).
 template <int NT, int VT, bool inclusive, typename T>
-__global__ __launch_bounds__(NT) void kernel_scan64_down(const T* input, T* output, int64_t count,
-                                                         const T* carries, T init) {
+__global__ __launch_bounds__(NT) void kernel_scan64_lookback(const T* input, T* output,
+                                                             int64_t count, T init,
+                                                             uint64_t* status,
+                                                             lookback_config_t lb) {
   constexpr int NV = NT * VT;
   constexpr int NW = NT / warp_size;
-  __shared__ T s_tile[NV];
-  __shared__ T s_wave[NW];
+  constexpr int NR = VT / 2;
+  static_assert(VT % 2 == 0, "VT must be even (pairs per row)");
+  __shared__ uint64_t s_wave[NR][NW];
+  __shared__ uint64_t s_carry;
   const int tid = threadIdx.x, lane = lane_id(), warp = tid / warp_size;
-  const int64_t base = (int64_t)blockIdx.x * NV;
-  // Coalesced load, transpose through LDS to VT consecutive per thread.
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * NV;
+  const bool full = count - base >= NV && 0 == ((uintptr_t)input & 15) &&
+                    0 == ((uintptr_t)output & 15);
+  typedef __attribute__((ext_vector_type(2))) uint64_t u64x2;
+  uint64_t x[VT];
+  if (full) {
+    const u64x2* in2 = (const u64x2*)(input + base);
 #pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    int64_t idx = base + i * NT + tid;
-    s_tile[i * NT + tid] = idx < count ? input[idx] : T(0);
+    for (int r = 0; r < NR; ++r) {
+      const u64x2 v = __builtin_nontemporal_load(in2 + r * NT + tid);
+      x[2 * r] = v.x;
+      x[2 * r + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int64_t idx = base + (int64_t)(r * NT + tid) * 2 + q;
+        x[2 * r + q] = idx < count ? (uint64_t)input[idx] : 0ull;
+      }
+  }
+  uint64_t w_excl[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    x[2 * r + 1] += x[2 * r];
+    uint64_t incl = x[2 * r + 1];
+#pragma unroll
+    for (int off = 1; off < warp_size; off <<= 1) {
+      const uint64_t y = __shfl_up(incl, off, warp_size);
+      if (lane >= off) incl += y;
+    }
+    if (lane == warp_size - 1) s_wave[r][warp] = incl;
+    w_excl[r] = wave_shift_up1(incl, (uint64_t)0);
   }
   __syncthreads();
-  T x[VT];
-  T run = 0;
-#pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    x[i] = s_tile[tid * VT + i];
-    run += x[i];
-  }
-  // Inclusive wave scan of the per-thread totals (64-bit, via shuffles).
-  T incl = run;
-#pragma unroll
-  for (int off = 1; off < warp_size; off <<= 1) {
-    T y = __shfl_up(incl, off, warp_size);
-    if (lane >= off) incl += y;
-  }
-  if (lane == warp_size - 1) s_wave[warp] = incl;
-  __syncthreads();
-  T carry = (carries ? carries[blockIdx.x] : T(0)) + init;
-  for (int w = 0; w < warp; ++w) carry += s_wave[w];
-  carry += incl - run;
-#pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    T v = x[i];
-    x[i] = inclusive ? carry + v : carry;
-    carry += v;
+  if (warp == 0) {
+    const uint64_t* flat = &s_wave[0][0];
+    uint64_t aggregate = 0;
+    for (int i = lane; i < NR * NW; i += warp_size) aggregate += flat[i];
+    aggregate = wave_reduce_all(aggregate);
+    uint64_t carry;
+    if (tile == 0) {
+      carry = (uint64_t)init;
+    } else {
+      if (lane == 0) {
+        scan64_publish(status, tile, aggregate, status_aggregate);
+        // The aggregate must be visible before this tile's output (possibly
+        // over its own input) can be: wait for the stores to complete.
+        __builtin_amdgcn_s_waitcnt(0);
+      }
+      carry = scan64_lookback<NV>(status, tile, (uint64_t)init, input, lb);
+    }
+    if (lane == 0) {
+      scan64_publish(status, tile, carry + aggregate, status_inclusive);
+      s_carry = carry;
+    }
   }
   __syncthreads();
+  // Outputs replace the inputs in x: x[2r] is the first element of the
+  // thread's pair, x[2r + 1] the pair's sum.
+  uint64_t run = s_carry;
 #pragma unroll
-  for (int i = 0; i < VT; ++i) s_tile[tid * VT + i] = x[i];
-  __syncthreads();
+  for (int r = 0; r < NR; ++r) {
+    uint64_t pre = run, tot = 0;
 #pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    int64_t idx = base + i * NT + tid;
-    if (idx < count) output[idx] = s_tile[i * NT + tid];
+    for (int w = 0; w < NW; ++w) {
+      const uint64_t v = s_wave[r][w];
+      if (w < warp) pre += v;
+      tot += v;
+    }
+    run += tot;
+    pre += w_excl[r];
+    const uint64_t first = x[2 * r], pair = x[2 * r + 1];
+    x[2 * r] = inclusive ? pre + first : pre;
+    x[2 * r + 1] = inclusive ? pre + pair : pre + first;
+  }
+  if (full) {
+    u64x2* out2 = (u64x2*)(output + base);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      u64x2 v;
+      v.x = x[2 * r];
+      v.y = x[2 * r + 1];
+      __builtin_nontemporal_store(v, out2 + r * NT + tid);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int64_t idx = base + (int64_t)(r * NT + tid) * 2 + q;
+        if (idx < count) output[idx] = (T)x[2 * r + q];
+      }
   }
 }
 
-template <typename T>
-inline size_t scan64_aux_bytes(int64_t count) {
-  constexpr int NV = 256 * 8;
-  size_t need = 0;
-  for (int64_t n = count; n > NV; n = div_up(n, (int64_t)NV)) need += aux_bytes<T>(div_up(n, (int64_t)NV));
-  return need;
-}
+constexpr int scan64_nt = 256, scan64_vt = 48;
 
-template <typename T>
-inline hipError_t scan64_run(void* p, const T* input, T* output, int64_t count, bool inclusive,
-                             T init, hipStream_t stream) {
-  constexpr int NT = 256, VT = 8, NV = NT * VT;
-  const int64_t tiles = div_up(count, (int64_t)NV);
-  hipError_t e;
-  if (tiles == 1) {
-    if (inclusive)
-      return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, true, T>, dim3(1),
-                           dim3(NT), 0, stream, input, output, count, (const T*)nullptr, init);
-    return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, false, T>, dim3(1),
-                         dim3(NT), 0, stream, input, output, count, (const T*)nullptr, init);
-  }
-  T* partials = (T*)p;
-  void* rest = (char*)p + aux_bytes<T>(tiles);
-  e = launch_kernel("mgpu_scan64_reduce", kernel_scan64_reduce<NT, VT, T>, dim3((unsigned)tiles),
-                    dim3(NT), 0, stream, input, count, partials);
-  if (e != hipSuccess) return e;
-  e = scan64_run<T>(rest, (const T*)partials, partials, tiles, false, T(0), stream);
-  if (e != hipSuccess) return e;
-  if (inclusive)
-    return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, true, T>,
-                         dim3((unsigned)tiles), dim3(NT), 0, stream, input, output, count,
-                         (const T*)partials, init);
-  return launch_kernel("mgpu_scan64_down", kernel_scan64_down<NT, VT, false, T>,
-                       dim3((unsigned)tiles), dim3(NT), 0, stream, input, output, count,
-                       (const T*)partials, init);
-}
-
-// Scan of 64-bit integers (uint64_t / int64_t, wrapping modulo 2^64).
+// Scan of 64-bit integers (uint64_t / int64_t, wrapping modulo 2^64); input
+// and output may alias.
 template <typename T>
 mgpu_status_t scan64(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
                      T* output, int64_t count, mgpu_scan_type_t type, T init) {
   static_assert(sizeof(T) == 8, "64-bit element types");
+  constexpr int NT = scan64_nt, VT = scan64_vt, NV = NT * VT;
   if (count < 0) return MGPU_INVALID_ARG;
   if (type != MGPU_SCAN_EXCLUSIVE && type != MGPU_SCAN_INCLUSIVE) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
-  if (div_up(count, (int64_t)2048) >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
-  const size_t need = scan64_aux_bytes<T>(count);
+  const int64_t num_tiles = div_up(count, (int64_t)NV);
+  if (num_tiles >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
+  const size_t need = aux_bytes<uint64_t>(2 * num_tiles);
   if (!aux_data) {
     aux_size += need;
     return MGPU_OK;
   }
   if (!input || !output) return MGPU_INVALID_ARG;
   if (aux_size < need) return MGPU_AUX_TOO_SMALL;
-  return hip_status(scan64_run<T>(aux_data, input, output, count, type == MGPU_SCAN_INCLUSIVE,
-                                  init, stream));
+  uint64_t* status = (uint64_t*)aux_data;
+  hipError_t e = fill_u32((uint32_t*)status, 4 * num_tiles, 0u, stream);
+  if (e != hipSuccess) return hip_status(e);
+  const lookback_config_t lb = lookback_config();
+  if (type == MGPU_SCAN_INCLUSIVE)
+    e = launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, true, T>,
+                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,
+                      status, lb);
+  else
+    e = launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, false, T>,
+                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,
+                      status, lb);
+  return hip_status(e);
 }
 
 }  // namespace hip

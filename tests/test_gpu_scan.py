@@ -162,19 +162,47 @@ def test_scan_forced_lookback_fallback(limit, inplace):
     assert np.max(np.abs(to_host(outf).astype(np.float64) - exact) / exact) <= F32_SCAN_RTOL
 
 
-@pytest.mark.parametrize("n", [1, 7, 2047, 2048, 2049, 2048 * 2048 + 3, (1 << 22) + 1])
+U64_NV = 256 * 48  # tile of the 64-bit scan
+
+
+def u64_expected(x, inclusive, init):
+    with np.errstate(over="ignore"):
+        inc = np.cumsum(x, dtype=np.uint64) + np.uint64(init)
+    return inc if inclusive else np.concatenate([np.array([init], np.uint64), inc[:-1]])
+
+
+@pytest.mark.parametrize("n", [1, 7, U64_NV - 1, U64_NV, U64_NV + 1, 3 * U64_NV + 5,
+                               300 * U64_NV + 1, (1 << 22) + 1])
 @pytest.mark.parametrize("inclusive", [False, True])
-def test_scan_u64_three_phase(n, inclusive):
+def test_scan_u64(n, inclusive):
     banner("scan u64", n, inclusive)
     x = oracle.generate(n, "u64", 30)
     d = to_dev(x)
     out = torch.empty_like(d)
     mgpu_hip.scan_into(d, out, inclusive=inclusive, init=12345)
-    with np.errstate(over="ignore"):
-        inc = np.cumsum(x, dtype=np.uint64) + np.uint64(12345)
-    exp = inc if inclusive else np.concatenate([np.array([12345], np.uint64), inc[:-1]])
-    np.testing.assert_array_equal(to_host(out), exp)
+    np.testing.assert_array_equal(to_host(out), u64_expected(x, inclusive, 12345))
     r = torch.zeros(1, dtype=torch.uint64, device="cuda")
     mgpu_hip.reduce(d, r, init=5)
     with np.errstate(over="ignore"):
         assert int(to_host(r)[0]) == int(np.sum(x, dtype=np.uint64) + np.uint64(5))
+
+
+@pytest.mark.parametrize("limit", [0, 1, 512])
+@pytest.mark.parametrize("in_place", [False, True])
+def test_scan_u64_lookback_fallback_and_in_place(limit, in_place):
+    """A tiny spin limit makes tiles sum late predecessors from their input
+    (in place: with the re-check of the predecessor's words); the result is
+    unchanged."""
+    banner("scan u64 fallback", limit, in_place)
+    n = 700 * U64_NV + 17
+    x = oracle.generate(n, "u64", 31)
+    old = mgpu_hip.lookback_spin_limit()
+    try:
+        mgpu_hip.set_lookback_spin_limit(limit)
+        d = to_dev(x)
+        out = d if in_place else torch.empty_like(d)
+        mgpu_hip.scan_into(d, out, inclusive=False, init=3)
+        torch.cuda.synchronize()
+    finally:
+        mgpu_hip.set_lookback_spin_limit(old)
+    np.testing.assert_array_equal(to_host(out), u64_expected(x, False, 3))
