@@ -273,7 +273,11 @@ def secondary(torch, mgpu_hip, steps, warmup, chart=True):
          "mgpu_reduce_partial"),
         ("segmented_scan_exclusive_i32_2p29_65536_segments",
          lambda aux: mgpu_hip.segmented_scan(xi, yi, segs, aux=aux), 8 * s, "mgpu_segscan"),
+        ("scan_exclusive_u64_2p27", lambda aux: mgpu_hip.scan_into(x64, y64, aux=aux),
+         16 * (s // 4), "mgpu_scan64_lookback"),
     ]
+    # 2^27 uint64 over the first half of the int32 input's bytes (a view).
+    x64, y64 = xi.view(torch.uint64)[: s // 4], yi.view(torch.uint64)[: s // 4]
     from mgpu_hip import workloads
     lens = workloads.lognormal_lengths(65536, 2048.0, 1.5, 1, 1 << 22, seed=0)
     starts = workloads.segment_starts(lens)
