@@ -6,7 +6,9 @@
 //                 2^28: 512x20 61.2 (512x16 57.7, 512x24 4.47 ms vs 4.36)
 //   32-bit keys:  2^22: 512x8 34.8, 512x12 33.2, 512x20 31.3; 16M: 512x12 67.7, 512x20 62.1;
 //                 25M: 512x12 75.3, 512x24 73.3; 2^28: 512x24 91.5, 512x20 89.6, 512x12 86.2
-//   64-bit kv:    1M: 512x8 5.5, 512x12 4.9; 16M: 512x12 18.4, 512x8 17.7
+//   64-bit kv:    1M: 512x8 5.5, 512x12 4.9; 16M: 512x12 18.4, 512x8 17.7;
+//                 2^27: 256x24 21.3, 512x12 19.8, 256x32 20.4, 512x16 19.4
+//   64-bit keys:  2^27: 512x12 29.8, 512x16 29.9, 256x24 29.9, 256x32 30.1
 #pragma once
 #include <type_traits>
 #include "mgpu/hip/radix.hxx"
@@ -21,6 +23,7 @@ inline mgpu_status_t radix_sort_tuned(void* aux, size_t& aux_size, hipStream_t s
   if constexpr (sizeof(key_t) == 8) {
     if (count <= ((int64_t)1 << 22))
       return radix_sort<512, 8, 512>(aux, aux_size, stream, keys, vals, count);
+    if constexpr (kv) return radix_sort<256, 24, 512>(aux, aux_size, stream, keys, vals, count);
     return radix_sort<512, 12, 512>(aux, aux_size, stream, keys, vals, count);
   } else if constexpr (kv) {
     if (count <= ((int64_t)6 << 20))
