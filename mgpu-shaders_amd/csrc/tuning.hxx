@@ -6,11 +6,14 @@
 // Merge / mergesort tiles, measured on MI355X with 32-bit keys: merge of
 // 2 x 2^27 keys 0.404 ms at 512 x 23 (256 x 15: 0.489); mergesort of 2^28
 // key-value pairs 13.6 ms at 512 x 19 (256 x 15: 16.2, 512 x 11: 14.3).
-// 8-byte keys keep 256 x 15 (the larger tiles would need > 64 KB of LDS).
+// 8-byte keys: merge of 2 x 2^26 u64 keys 0.395 ms at 512 x 15 (256 x 15:
+// 0.435, 256 x 23: 0.421, 512 x 11: 0.427); mergesort of 2^27 u64 pairs
+// 9.80 ms at 512 x 13 (512 x 11: 9.87, 512 x 9: 10.34, 256 x 15: 10.65,
+// 512 x 15: 12.08).
 template <typename key_t, int bytes = sizeof(key_t)> struct merge_tuning { static constexpr int nt = 512, vt = 23; };
-template <typename key_t> struct merge_tuning<key_t, 8> { static constexpr int nt = 256, vt = 15; };
+template <typename key_t> struct merge_tuning<key_t, 8> { static constexpr int nt = 512, vt = 15; };
 template <typename key_t, int bytes = sizeof(key_t)> struct mergesort_tuning { static constexpr int nt = 512, vt = 19; };
-template <typename key_t> struct mergesort_tuning<key_t, 8> { static constexpr int nt = 256, vt = 15; };
+template <typename key_t> struct mergesort_tuning<key_t, 8> { static constexpr int nt = 512, vt = 13; };
 // Scan: 256 x 80 (20480-element tiles).  Larger tiles mean fewer look-back
 // steps per byte; 2^29 int32 exclusive: 0.775 ms at 256 x 80 vs 1.19 ms at
 // 256 x 16 on MI355X (float32 inclusive 0.822 vs 1.26 ms).

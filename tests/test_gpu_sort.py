@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 KTS = ("u32", "i32", "u64", "f32", "i64", "f64")
 MS_NV = 512 * 19      # mergesort tile of 32-bit keys in the C ABI instantiation
-MS_NV64 = 256 * 15    # mergesort and merge tile of 64-bit keys
+MS_NV64 = 512 * 13    # mergesort tile of 64-bit keys
+MERGE_NV64 = 512 * 15  # merge tile of 64-bit keys
 MERGE_NV = 512 * 23   # merge tile of 32-bit keys
 RADIX_NV = 512 * 8    # radix tile of every size below 2^22 keys
 SIZES = [1, 2, 7, 64, 100, MS_NV64 - 1, MS_NV64, MS_NV64 + 1, MS_NV - 1, MS_NV, MS_NV + 1,
@@ -115,6 +116,7 @@ def test_mergesort_vk_sort_known_answer():
 # ---------------------------------------------------------------------- merge
 
 MERGE_SIZES = [(0, 1), (1, 0), (1, 1), (5, 3000), (3000, 5), (MS_NV64, MS_NV64),
+               (MERGE_NV64 // 2, MERGE_NV64 // 2 + 1), (MERGE_NV64, 1),
                (MERGE_NV // 2, MERGE_NV // 2), (MERGE_NV, 1), (10007, 20011),
                (1 << 19, (1 << 19) + 3)]
 
