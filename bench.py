@@ -229,7 +229,8 @@ def secondary(torch, mgpu_hip, steps, warmup, chart=True):
         del ks, k, aux
     if rows:
         out["radix_keys_u32_reference_chart_sizes"] = {
-            "rows": rows, "algorithm": "automatic (single pass up to 2^22 keys)",
+            "rows": rows,
+            "algorithm": "count + scan + scatter per digit; tile shape by key count",
             "min_ratio_vs_chart_high": min(r["ratio_vs_chart_high"] for r in rows)}
         log("radix key-only at chart sizes",
             [(r["keys"] // 1000000, round(r["mkeys_per_s"]), round(r["ratio_vs_chart_high"], 1))

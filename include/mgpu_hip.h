@@ -145,8 +145,9 @@ MGPU_DECL_RADIX(f64, double)
 MGPU_DECL_SCAN(i32, int32_t)
 MGPU_DECL_SCAN(u32, uint32_t)
 MGPU_DECL_SCAN(f32, float)
-/* u64 (modulo 2^64): reduce-then-scan in three phases (vk/scan.hxx:65-113),
- * since a 64-bit sum and a look-back flag do not fit one atomic word.      */
+/* u64 (modulo 2^64): single pass with look-back like the 32-bit scans; a
+ * tile's 64-bit sum and flag are published as two atomic words, each with
+ * the flag beside one 32-bit half (vk/scan.hxx:10-115 counterpart).        */
 MGPU_DECL_SCAN(u64, uint64_t)
 
 /* Segmented scan (no counterpart in the reference, whose scan has no

@@ -17,7 +17,7 @@
 //   3. per bucket one persistent kernel: each block sorts whole segments in
 //      LDS (the blocksort of mergesort.hxx: odd-even network + in-LDS merge
 //      passes) and takes the next segment of its bucket;
-//   4. large segments: a segmented LSD radix sort over 12288-key tiles that
+//   4. large segments: a segmented LSD radix sort over 8192-key tiles that
 //      never straddle segments.  Counts are laid out [segment][digit][tile],
 //      so one exclusive scan gives every key's position in the concatenation
 //      of the large segments; position - (start of its segment in that
