@@ -403,6 +403,9 @@ __global__ __launch_bounds__(NT, 2 * NT / 256) void kernel_radix_scatter(
 // per round (coalesced across digits), spins on an unpublished word at most
 // lb.spin_limit times and then counts that predecessor's digit from its
 // input, which a ping-pong pass never overwrites.
+// Used only when selected (radix_algorithm_setting() == 2): on MI355X the
+// look-back over status words exchanged across XCDs costs more per pass than
+// the count kernel and scan it removes, at every size measured.
 
 constexpr uint32_t radix_word_agg = 1u << 30, radix_word_inc = 2u << 30;
 constexpr uint32_t radix_word_value = (1u << 30) - 1;
