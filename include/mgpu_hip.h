@@ -19,10 +19,16 @@
  * find_log2(0) is undefined, inc/mgpu/common/meta.hxx:37-41).
  *
  * All pointers are device pointers (hipMalloc / torch tensors).  Vectorised
- * 128-bit paths are taken when pointers are 16-byte aligned, otherwise a
- * scalar path runs; results are identical.  Threading: the library holds no
- * mutable global state on the compute path; calls on different streams are
- * independent.  No C++ exception crosses this boundary.
+ * 128-bit paths are taken when data pointers are 16-byte aligned, otherwise a
+ * scalar path runs; results are identical.  aux itself must be 256-byte
+ * aligned, as hipMalloc returns it, whenever the call needs any aux bytes
+ * (MGPU_INVALID_ARG otherwise); every requirement is a multiple of 256
+ * bytes, so consecutive regions of one pool sized by accumulation stay
+ * aligned.  Threading: calls on different streams are independent and may
+ * come from different host threads; the only global state (the radix
+ * algorithm setting, an atomic; the segmented-sort tuning table and the
+ * serving engine, each under a mutex) is safe to share.  No C++ exception
+ * crosses this boundary.
  */
 #ifndef MGPU_HIP_H
 #define MGPU_HIP_H
@@ -39,9 +45,10 @@ typedef struct ihipStream_t* mgpu_stream_t;
 
 typedef enum mgpu_status_t {
   MGPU_OK = 0,
-  MGPU_INVALID_ARG = 1,   /* negative count, bad enum; NULL data pointer with
-                             count > 0 on the call that enqueues (the sizing
-                             call with aux == NULL never reads data pointers) */
+  MGPU_INVALID_ARG = 1,   /* negative count, bad enum, aux not 256-byte
+                             aligned, or a NULL data pointer with count > 0
+                             on the call that enqueues (the sizing call with
+                             aux == NULL never reads data pointers)          */
   MGPU_AUX_TOO_SMALL = 2, /* *aux_size (capacity) below the requirement          */
   MGPU_HIP_ERROR = 3,     /* a HIP runtime call failed; see mgpu_last_hip_error  */
   MGPU_UNSUPPORTED = 4    /* size beyond a documented limit of the entry point   */

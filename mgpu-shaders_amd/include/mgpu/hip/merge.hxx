@@ -213,7 +213,7 @@ mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
   if ((a_count && !a_keys) || (b_count && !b_keys) || !c_keys) return MGPU_INVALID_ARG;
   if (has_values && ((a_count && !a_vals) || (b_count && !b_vals) || !c_vals))
     return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   void* p = aux_data;
   int64_t* mp = advance_pointer<int64_t>(p, num_partitions);
   constexpr int PNT = 256;

@@ -266,7 +266,7 @@ mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream,
     return MGPU_OK;
   }
   if (!keys || (has_values && !vals)) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   hipError_t e;
   if (num_passes == 0) {
     e = launch_kernel("mgpu_blocksort", kernel_blocksort<NT, VT, sort_indices, key_t, val_t, comp_t>,

@@ -117,6 +117,15 @@ __host__ inline T* advance_pointer(void*& p, int64_t count) {
   p = (char*)p + aux_bytes<T>(count);
   return r;
 }
+// The enqueueing call's check of the caller's aux: capacity first, then
+// alignment.  Regions are carved at 256-byte offsets from aux and hold
+// status words used with 64-bit atomics and 128-bit loads, so aux must be
+// 256-byte aligned (as hipMalloc returns) whenever any bytes are needed.
+__host__ inline mgpu_status_t check_aux(const void* aux, size_t aux_size, size_t need) {
+  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (need && (uintptr_t)aux % aux_align) return MGPU_INVALID_ARG;
+  return MGPU_OK;
+}
 
 ////////////////////////////////////////////////////////////////////////////////
 // Per-thread last HIP error (reported through mgpu_last_hip_error).

@@ -635,7 +635,7 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
     return MGPU_OK;
   }
   if (!keys || (has_values && !vals)) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   void* p = aux_data;
   key_t* keys2 = advance_pointer<key_t>(p, count);
   uint32_t* vals2 = has_values ? advance_pointer<uint32_t>(p, count) : nullptr;

@@ -306,7 +306,7 @@ mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t strea
     return MGPU_OK;
   }
   if (!input || !output) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   void* p = aux_data;
   uint64_t* status = advance_pointer<uint64_t>(p, num_tiles);
   hipError_t e = fill_u32((uint32_t*)status, 2 * num_tiles, 0u, stream);
@@ -421,7 +421,7 @@ mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const
     return MGPU_OK;
   }
   if (!out || (count > 0 && !input)) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   void* p = aux_data;
   acc_t* partials = advance_pointer<acc_t>(p, blocks);
   int64_t mis = ((uintptr_t)input & 15) / sizeof(T);
@@ -653,7 +653,7 @@ mgpu_status_t scan64(void* aux_data, size_t& aux_size, hipStream_t stream, const
     return MGPU_OK;
   }
   if (!input || !output) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   uint64_t* status = (uint64_t*)aux_data;
   hipError_t e = fill_u32((uint32_t*)status, 4 * num_tiles, 0u, stream);
   if (e != hipSuccess) return hip_status(e);

@@ -384,7 +384,7 @@ mgpu_status_t segmented_transform_scan(void* aux_data, size_t& aux_size, hipStre
     return MGPU_OK;
   }
   if (!input || !output || (num_segments > 0 && !segments)) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   void* p = aux_data;
   uint64_t* status = advance_pointer<uint64_t>(p, num_tiles);
   int32_t* tile_first = advance_pointer<int32_t>(p, num_tiles + 1);

@@ -300,7 +300,7 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
     return MGPU_OK;
   }
   if (!keys || !segments || (has_values && !vals)) return MGPU_INVALID_ARG;
-  if (aux_size < need) return MGPU_AUX_TOO_SMALL;
+  if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
 
   const int NB = tuning.num_buckets;
   const int64_t S = num_segments, B = NB + 1;

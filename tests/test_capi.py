@@ -167,6 +167,35 @@ def test_aux_too_small_is_reported_before_any_launch():
     assert st == mgpu_hip.MGPU_AUX_TOO_SMALL
 
 
+def test_misaligned_aux_is_rejected_before_any_launch():
+    """aux is carved at 256-byte offsets and holds atomically updated status
+    words, so an aux that is not 256-byte aligned is an argument error (after
+    the capacity check).  Data pointers are never dereferenced here."""
+    lib = mgpu_hip.load()
+    one = ctypes.c_void_p(1)
+    seg = ctypes.c_void_p(256)
+    calls = [
+        (lib.mgpu_mergesort_keys_u32, (one, 1 << 20, 0)),
+        (lib.mgpu_merge_keys_u32, (one, 1 << 20, one, 1 << 20, one, 0)),
+        (lib.mgpu_radix_sort_kv_u32_u32, (one, one, 1 << 20)),
+        (lib.mgpu_scan_i32, (one, one, 1 << 20, 0, 0)),
+        (lib.mgpu_reduce_i32, (one, 1 << 20, one, 0)),
+        (lib.mgpu_scan_u64, (one, one, 1 << 20, 0, 0)),
+        (lib.mgpu_segmented_scan_i32, (one, one, 1 << 20, seg, 100, 0, 0)),
+        (lib.mgpu_segmented_sort_keys_u32, (one, 1 << 20, seg, 100, 0, None)),
+    ]
+    for fn, args in calls:
+        st, need = _size(fn, *args)
+        assert st == mgpu_hip.MGPU_OK and need > 0, fn
+        for base in (0x100000 + 4, 0x100000 + 128):
+            cap = ctypes.c_size_t(need)
+            assert fn(ctypes.c_void_p(base), ctypes.byref(cap), None, *args) == \
+                mgpu_hip.MGPU_INVALID_ARG, (fn, base)
+            cap = ctypes.c_size_t(16)
+            assert fn(ctypes.c_void_p(base), ctypes.byref(cap), None, *args) == \
+                mgpu_hip.MGPU_AUX_TOO_SMALL, (fn, base)
+
+
 def test_python_surface_rejects_host_tensors():
     import torch
     with pytest.raises(ValueError):
