@@ -4,7 +4,8 @@ through size-independent properties:
   the input (sum, and sums of products of the 16-bit halves);
 * exclusive scan of 2^31 + 7 int32 in [0, 16): out[0] = init and
   out[i + 1] = out[i] + x[i] (mod 2^32) for every i, which fixes every output;
-* merge of two sorted 2^30-key arrays: ordered, and the multiset of both."""
+* merge of two sorted 2^30-key arrays: ordered, and the multiset of both;
+* key-value mergesort of 2^31 - 1 pairs, its largest accepted count."""
 import pytest
 import torch
 
@@ -66,6 +67,44 @@ def test_exclusive_scan_above_2p31():
         assert bool((d == 0).all()), "out[i+1] != out[i] + x[i] near %d" % i
         del d
     del x, out
+    torch.cuda.empty_cache()
+
+
+def test_mergesort_kv_at_count_limit():
+    """Key-value mergesort of 2^31 - 1 pairs (the largest count the entry
+    point accepts), keys in [0, 2^20) so that nearly every key has ties and
+    values = input index.  Checked in chunks: keys ordered; values a
+    permutation of [0, n) (every slot of a bitmap hit); keys_out[i] ==
+    keys_in[vals_out[i]]; equal keys keep ascending values.  Together these
+    fix the stable sort uniquely."""
+    n = (1 << 31) - 1
+    banner("mergesort kv at the count limit", n)
+    keys = gen_dev(n, "u32", 45)
+    keys.view(torch.int32).bitwise_and_(0xFFFFF)
+    keys_in = keys.clone()
+    vals = gen_dev(n, "iota", 0)
+    mgpu_hip.mergesort_kv(keys, vals)
+    torch.cuda.synchronize()
+    assert_ordered(keys)
+    seen = torch.zeros(n, dtype=torch.uint8, device=keys.device)
+    kin = keys_in.view(torch.int32)
+    for i in range(0, n, CHUNK):
+        j = min(i + CHUNK, n)
+        v = vals[i:j].view(torch.int32).long()
+        seen[v] = 1
+        assert bool((kin[v] == keys.view(torch.int32)[i:j]).all()), \
+            "keys_out != keys_in[vals_out] near %d" % i
+        del v
+    assert bool(seen.all()), "values are not a permutation"
+    del seen, keys_in, kin
+    for i in range(0, n - 1, CHUNK):
+        j = min(i + CHUNK + 1, n)
+        k = keys.view(torch.int32)[i:j]
+        v = vals.view(torch.int32)[i:j]
+        tie = k[1:] == k[:-1]
+        assert bool((v[1:][tie] > v[:-1][tie]).all()), "ties out of input order near %d" % i
+        del k, v, tie
+    del keys, vals
     torch.cuda.empty_cache()
 
 
