@@ -1,6 +1,7 @@
 """GPU parity above 2^31 elements, where 32-bit index arithmetic would wrap,
 through size-independent properties:
-* radix sort of 2^31 + 12345 uint32 keys: ordered, and the same multiset as
+* radix sort of 2^31 + 12345 and of 2^32 - 12289 (its count limit) uint32
+  keys: ordered, and the same multiset as
   the input (sum, and sums of products of the 16-bit halves);
 * exclusive scan of 2^31 + 7 int32 in [0, 16): out[0] = init and
   out[i + 1] = out[i] + x[i] (mod 2^32) for every i, which fixes every output;
@@ -43,6 +44,22 @@ def test_radix_keys_above_2p31():
     n = (1 << 31) + 12345
     banner("radix keys", n)
     keys = gen_dev(n, "u32", 41)
+    before = multiset_signature(keys)
+    mgpu_hip.radix_sort(keys)
+    torch.cuda.synchronize()
+    assert_ordered(keys)
+    assert multiset_signature(keys) == before
+    del keys
+    torch.cuda.empty_cache()
+
+
+def test_radix_keys_at_count_limit():
+    """2^32 - 12289 keys, the largest count the 32-bit radix entry points
+    accept (output positions are 32-bit, less one 12288-key tile): ordered,
+    and the same multiset as the input."""
+    n = (1 << 32) - 12289
+    banner("radix keys at the count limit", n)
+    keys = gen_dev(n, "u32", 46)
     before = multiset_signature(keys)
     mgpu_hip.radix_sort(keys)
     torch.cuda.synchronize()
