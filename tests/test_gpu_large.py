@@ -6,7 +6,8 @@ through size-independent properties:
 * exclusive scan of 2^31 + 7 int32 in [0, 16): out[0] = init and
   out[i + 1] = out[i] + x[i] (mod 2^32) for every i, which fixes every output;
 * merge of two sorted 2^30-key arrays: ordered, and the multiset of both;
-* key-value mergesort of 2^31 - 1 pairs, its largest accepted count."""
+* key-value mergesort, segmented scan and key-value segmented sort at
+  2^31 - 1 elements, their largest accepted count."""
 import pytest
 import torch
 
@@ -122,6 +123,84 @@ def test_mergesort_kv_at_count_limit():
         assert bool((v[1:][tie] > v[:-1][tie]).all()), "ties out of input order near %d" % i
         del k, v, tie
     del keys, vals
+    torch.cuda.empty_cache()
+
+
+def random_starts(n, count, seed):
+    """Ascending distinct segment starts in [0, n), element 0 always one."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    s = torch.unique(torch.randint(1, n, (count - 1,), generator=g, device="cuda"))
+    return torch.cat([torch.zeros(1, dtype=torch.int64, device="cuda"), s]).to(torch.int32)
+
+
+def start_mask(n, segs):
+    m = torch.zeros(n, dtype=torch.bool, device="cuda")
+    m[segs.long()] = True
+    return m
+
+
+def test_segmented_scan_at_count_limit():
+    """Exclusive segmented scan of 2^31 - 1 int32 in [0, 16) over ~65536
+    random segments, init 3: out[s] = 3 at every segment start s and
+    out[p] = out[p - 1] + x[p - 1] everywhere else, which fixes every output."""
+    n = (1 << 31) - 1
+    banner("segmented scan at the count limit", n)
+    x = gen_dev(n, "i32_0_16", 47)
+    segs = random_starts(n, 1 << 16, 47)
+    out = torch.empty_like(x)
+    mgpu_hip.segmented_scan(x, out, segs, inclusive=False, init=3)
+    torch.cuda.synchronize()
+    start = start_mask(n, segs)
+    assert bool((out[segs.long()] == 3).all()), "segment start != init"
+    for i in range(1, n, CHUNK):
+        j = min(i + CHUNK, n)
+        d = out[i:j] - out[i - 1:j - 1] - x[i - 1:j - 1]
+        assert bool((d[~start[i:j]] == 0).all()), "out[p] != out[p-1] + x[p-1] near %d" % i
+        del d
+    del x, out, start
+    torch.cuda.empty_cache()
+
+
+def test_segmented_sort_kv_at_count_limit():
+    """Key-value segmented sort of 2^31 - 1 pairs over ~2^18 random segments
+    (lengths from 1 to a few times 8192, so every size bucket and the
+    large-segment radix path take part), keys in [0, 2^16) for ties, values =
+    input index.  Checked in chunks: values a permutation of [0, n), each in
+    the segment of its output position; keys_out[i] == keys_in[vals_out[i]];
+    inside a segment keys ascend and equal keys keep ascending values.
+    Together these fix every segment's stable sort."""
+    n = (1 << 31) - 1
+    banner("segmented sort kv at the count limit", n)
+    keys = gen_dev(n, "u32", 48)
+    keys.view(torch.int32).bitwise_and_(0xFFFF)
+    keys_in = keys.clone()
+    vals = gen_dev(n, "iota", 0)
+    segs = random_starts(n, 1 << 18, 48)
+    mgpu_hip.segmented_sort(keys, segs, vals)
+    torch.cuda.synchronize()
+    ki, ko, vo = keys_in.view(torch.int32), keys.view(torch.int32), vals.view(torch.int32)
+    sl = segs.long()
+    seen = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    for i in range(0, n, CHUNK):
+        j = min(i + CHUNK, n)
+        v = vo[i:j].long()
+        seen[v] = 1
+        assert bool((ki[v] == ko[i:j]).all()), "keys_out != keys_in[vals_out] near %d" % i
+        pos = torch.arange(i, j, device="cuda")
+        assert bool((torch.searchsorted(sl, v, right=True) ==
+                     torch.searchsorted(sl, pos, right=True)).all()), \
+            "a value left its segment near %d" % i
+        del v, pos
+    assert bool(seen.all()), "values are not a permutation"
+    del seen, keys_in, ki
+    start = start_mask(n, segs)
+    for i in range(0, n - 1, CHUNK):
+        j = min(i + CHUNK + 1, n)
+        k, v = ko[i:j], vo[i:j]
+        ok = (k[1:] > k[:-1]) | ((k[1:] == k[:-1]) & (v[1:] > v[:-1]))
+        assert bool(ok[~start[i + 1:j]].all()), "segment not stably sorted near %d" % i
+        del k, v, ok
+    del keys, vals, start
     torch.cuda.empty_cache()
 
 
