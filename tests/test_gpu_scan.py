@@ -105,7 +105,9 @@ def test_scan_determinism_integers():
     assert dev_equal(a, b)
 
 
-@pytest.mark.parametrize("n", [1, 3, 4, 1000, 4097, 1 << 20, (1 << 22) + 13])
+# Reduce tiles hold 8192 four-byte elements (256 threads x 8 x 128-bit).
+@pytest.mark.parametrize("n", [1, 3, 4, 1000, 4097, 8191, 8192, 8195, 3 * 8192 + 5, 1 << 20,
+                               (1 << 22) + 13])
 @pytest.mark.parametrize("offset", [0, 1])
 def test_reduce(n, offset):
     banner("reduce", n, offset)
