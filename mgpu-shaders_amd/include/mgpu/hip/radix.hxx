@@ -15,10 +15,12 @@
 //      counts[d * tiles + t] becomes the global position of tile t's first key
 //      of digit d: sum_{d'<d} total(d') + sum_{t'<t} count(t', d), the
 //      (bin, tile) order of vk/radix.hxx:357-375;
-//   3. kernel_radix_scatter: ranks every key within its tile (wave64 match
-//      from 8 ballots on 64-bit lane masks + per-wave counters in LDS),
-//      reorders the tile by digit in LDS, and writes keys and values in runs
-//      of consecutive addresses (16 bytes moved per key-value pair).
+//   3. kernel_radix_scatter: ranks every key within its tile (a wave64 match
+//      of 64-bit lane masks, through an LDS table of per-digit masks for
+//      32-bit keys and as the AND of 8 digit-bit ballots for 64-bit keys, plus
+//      per-wave counters in LDS), reorders the tile by digit in LDS, and
+//      writes keys and values in runs of consecutive addresses (16 bytes
+//      moved per 32-bit key-value pair).
 // A single-pass structure (kernel_radix_onesweep) is available by setting;
 // on MI355X it is slower at every size (see radix_algorithm_setting).
 // Tiles are remapped so that each XCD (blockIdx % 8) owns a contiguous tile
