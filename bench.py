@@ -25,9 +25,10 @@ The JSON line also carries:
                 reference's published radix chart (10^6 x {1 ... 50} keys);
   vendor_rocprim  AMD's rocPRIM (compiled from ROCm's source headers into
                 vendor_rocprim.so) on the same inputs in the same process:
-                radix sort and merge sort of the 2^28 pairs, merge, scans,
-                reduce and the segmented sort layout, with each pair of
-                outputs compared where the result is unique;
+                radix sort and merge sort of the 2^28 pairs, radix sort of
+                2^27 u64 pairs and of 10^6 / 5 x 10^7 u32 keys, merge, scans
+                (i32, f32, u64), reduce and the segmented sort layout, with
+                each pair of outputs compared where the result is unique;
   serving       the request-serving engine at saturation and three loads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
@@ -400,6 +401,10 @@ def bench_vendor(torch, mgpu_hip, steps, warmup):
     sz = ctypes.POINTER(ctypes.c_size_t)
     sig = {"vendor_radix_sort_pairs_u32": [vp, sz, vp, vp, vp, vp, i64, vp,
                                            ctypes.POINTER(ctypes.c_int)],
+           "vendor_radix_sort_pairs_u64": [vp, sz, vp, vp, vp, vp, i64, vp,
+                                           ctypes.POINTER(ctypes.c_int)],
+           "vendor_radix_sort_keys_u32": [vp, sz, vp, vp, i64, vp, ctypes.POINTER(ctypes.c_int)],
+           "vendor_exclusive_scan_u64": [vp, sz, vp, vp, i64, ctypes.c_uint64, vp],
            "vendor_merge_sort_pairs_u32": [vp, sz, vp, vp, vp, vp, i64, vp],
            "vendor_merge_u32": [vp, sz, vp, i64, vp, i64, vp, vp],
            "vendor_exclusive_scan_i32": [vp, sz, vp, vp, i64, ctypes.c_int32, vp],
@@ -493,6 +498,53 @@ def bench_vendor(torch, mgpu_hip, steps, warmup):
         "keys_per_s")
     del temp, ks, vs, k, v, k2, v2
 
+    # 64-bit keys: radix sort of 2^27 (u64 key, u32 index) pairs.
+    n64 = 1 << 27
+    ks64 = torch.empty(n64, dtype=torch.uint64, device=dev)
+    mgpu_hip.generate(ks64, "u64", 0)
+    vs64 = torch.empty(n64, dtype=torch.uint32, device=dev)
+    mgpu_hip.generate(vs64, "iota")
+    k64, v64 = torch.empty_like(ks64), torch.empty_like(vs64)
+    k64b, v64b = torch.empty_like(ks64), torch.empty_like(vs64)
+
+    def restore64():
+        k64.copy_(ks64)
+        v64.copy_(vs64)
+    restore64()
+    aux = mgpu_hip.radix_sort(k64, v64)
+    ours_ms, _ = timed_steps(torch, steps, warmup, restore64,
+                             lambda: mgpu_hip.radix_sort(k64, v64, aux=aux))
+    ok, ov = k64.clone(), v64.clone()
+    del aux
+    run, temp = two_phase(lib.vendor_radix_sort_pairs_u64,
+                          (p(k64), p(k64b), p(v64), p(v64b), n64), (ctypes.byref(alt),))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, restore64, run)
+    torch.cuda.synchronize()
+    rk, rv = (k64b, v64b) if alt.value else (k64, v64)
+    row("radix_sort_kv_u64_2p27", ours_ms, vendor_ms, same(ok, rk) and same(ov, rv), n64,
+        "keys_per_s")
+    del temp, ok, ov, ks64, vs64, k64, v64, k64b, v64b
+
+    # Key-only 32-bit radix sort at the reference chart's smallest and
+    # largest sizes (10^6 and 5 x 10^7 keys).
+    for cnt in (1000000, 50000000):
+        kc = torch.empty(cnt, dtype=torch.uint32, device=dev)
+        mgpu_hip.generate(kc, "u32", 4)
+        kk, kb = torch.empty_like(kc), torch.empty_like(kc)
+        rst = lambda: kk.copy_(kc)
+        rst()
+        aux = mgpu_hip.radix_sort(kk)
+        ours_ms, _ = timed_steps(torch, max(steps, 20), warmup, rst,
+                                 lambda: mgpu_hip.radix_sort(kk, aux=aux))
+        ok = kk.clone()
+        run, temp = two_phase(lib.vendor_radix_sort_keys_u32, (p(kk), p(kb), cnt),
+                              (ctypes.byref(alt),))
+        vendor_ms, _ = timed_steps(torch, max(steps, 20), warmup, rst, run)
+        torch.cuda.synchronize()
+        row("radix_sort_keys_u32_%dM" % (cnt // 1000000), ours_ms, vendor_ms,
+            same(ok, kb if alt.value else kk), cnt, "keys_per_s")
+        del kc, kk, kb, aux, ok, temp
+
     # Merge of two sorted 2^27-key arrays.
     m = 1 << 27
     a = torch.empty(m, dtype=torch.uint32, device=dev)
@@ -534,7 +586,18 @@ def bench_vendor(torch, mgpu_hip, steps, warmup):
     vendor_ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
     torch.cuda.synchronize()
     row("reduce_i32_2p29", ours_ms, vendor_ms, same(r, r2), 4 * s, "bytes_per_s")
-    del aux, temp, yi, yi2, xi
+    del aux, temp
+    # 2^27 uint64 over the int32 input's bytes (a view), exclusive.
+    x64, y64, y64b = xi.view(torch.uint64), yi.view(torch.uint64), yi2.view(torch.uint64)
+    aux = mgpu_hip.scan_into(x64, y64)
+    ours_ms, _ = timed_steps(torch, steps, warmup, lambda: None,
+                             lambda: mgpu_hip.scan_into(x64, y64, aux=aux))
+    run, temp = two_phase(lib.vendor_exclusive_scan_u64, (p(x64), p(y64b), s // 2, 0))
+    vendor_ms, _ = timed_steps(torch, steps, warmup, lambda: None, run)
+    torch.cuda.synchronize()
+    row("scan_exclusive_u64_2p28", ours_ms, vendor_ms, same(y64, y64b), 16 * (s // 2),
+        "bytes_per_s")
+    del aux, temp, x64, y64, y64b, yi, yi2, xi
     xf = torch.empty(s, dtype=torch.float32, device=dev)
     mgpu_hip.generate(xf, "f32_01", 0)
     yf, yf2 = torch.empty_like(xf), torch.empty_like(xf)

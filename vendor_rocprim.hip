@@ -37,6 +37,35 @@ hipError_t vendor_radix_sort_pairs_u32(void* temp, size_t* temp_bytes, uint32_t*
   return e;
 }
 
+// The same for 64-bit keys (all 64 bits).
+hipError_t vendor_radix_sort_pairs_u64(void* temp, size_t* temp_bytes, uint64_t* keys,
+                                       uint64_t* keys_alt, uint32_t* vals, uint32_t* vals_alt,
+                                       int64_t n, hipStream_t stream, int* in_alt) {
+  rocprim::double_buffer<uint64_t> k(keys, keys_alt);
+  rocprim::double_buffer<uint32_t> v(vals, vals_alt);
+  hipError_t e = rocprim::radix_sort_pairs(temp, *temp_bytes, k, v, (size_t)n, 0u, 64u, stream);
+  if (in_alt) *in_alt = k.current() == keys_alt ? 1 : 0;
+  return e;
+}
+
+// Keys only, 32-bit, double buffer.
+hipError_t vendor_radix_sort_keys_u32(void* temp, size_t* temp_bytes, uint32_t* keys,
+                                      uint32_t* keys_alt, int64_t n, hipStream_t stream,
+                                      int* in_alt) {
+  rocprim::double_buffer<uint32_t> k(keys, keys_alt);
+  hipError_t e = rocprim::radix_sort_keys(temp, *temp_bytes, k, (size_t)n, 0u, 32u, stream);
+  if (in_alt) *in_alt = k.current() == keys_alt ? 1 : 0;
+  return e;
+}
+
+// Exclusive scan of uint64 (modulo 2^64) with an initial value, out of place.
+hipError_t vendor_exclusive_scan_u64(void* temp, size_t* temp_bytes, const uint64_t* in,
+                                     uint64_t* out, int64_t n, uint64_t init,
+                                     hipStream_t stream) {
+  return rocprim::exclusive_scan(temp, *temp_bytes, in, out, init, (size_t)n,
+                                 rocprim::plus<uint64_t>(), stream);
+}
+
 // Stable merge sort of (key, value) pairs with operator<, out of place.
 hipError_t vendor_merge_sort_pairs_u32(void* temp, size_t* temp_bytes, const uint32_t* keys_in,
                                        uint32_t* keys_out, const uint32_t* vals_in,
