@@ -230,10 +230,21 @@ def mergesort_indices(keys, indices, comp="less", key_type=None, aux=None, strea
 # ---------------------------------------------------------------------------
 # Merge (inc/mgpu/vk/merge.hxx:12-59)
 
-def merge_keys(a_keys, b_keys, c_keys, comp, key_type=None, aux=None, stream=None):
-    _device_check(a_keys, b_keys, c_keys)
+def _check_merge_shapes(a_keys, b_keys, c_keys, a_vals=None, b_vals=None, c_vals=None):
+    """The C entry points take raw pointers and counts from a and b: c and the
+    value arrays must hold exactly what those counts imply."""
     if c_keys.numel() != a_keys.numel() + b_keys.numel():
         raise ValueError("c must hold a + b elements")
+    if not a_keys.dtype == b_keys.dtype == c_keys.dtype:
+        raise ValueError("a, b and c keys must have one dtype")
+    for keys, vals, name in ((a_keys, a_vals, "a"), (b_keys, b_vals, "b"), (c_keys, c_vals, "c")):
+        if vals is not None and (vals.numel() != keys.numel() or vals.element_size() != 4):
+            raise ValueError("%s_vals must be 32-bit with one per %s key" % (name, name))
+
+
+def merge_keys(a_keys, b_keys, c_keys, comp, key_type=None, aux=None, stream=None):
+    _check_merge_shapes(a_keys, b_keys, c_keys)
+    _device_check(a_keys, b_keys, c_keys)
     k = _key_type(a_keys, key_type)
     fn = getattr(load(), "mgpu_merge_keys_%s" % k)
     return _run_two_phase(fn, [_ptr(a_keys), a_keys.numel(), _ptr(b_keys), b_keys.numel(),
@@ -242,9 +253,8 @@ def merge_keys(a_keys, b_keys, c_keys, comp, key_type=None, aux=None, stream=Non
 
 def merge(a_keys, a_vals, b_keys, b_vals, c_keys, c_vals, comp, key_type=None, aux=None,
           stream=None):
+    _check_merge_shapes(a_keys, b_keys, c_keys, a_vals, b_vals, c_vals)
     _device_check(a_keys, a_vals, b_keys, b_vals, c_keys, c_vals)
-    if c_keys.numel() != a_keys.numel() + b_keys.numel():
-        raise ValueError("c must hold a + b elements")
     k = _key_type(a_keys, key_type)
     fn = getattr(load(), "mgpu_merge_kv_%s_u32" % k)
     return _run_two_phase(fn, [_ptr(a_keys), _ptr(a_vals), a_keys.numel(), _ptr(b_keys),

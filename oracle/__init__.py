@@ -121,6 +121,8 @@ def glibc_rand(n):
 def stable_sort(keys, vals=None, kt="u32", greater=False):
     k = np.ascontiguousarray(keys, dtype=NP[kt]).copy()
     v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    if v is not None and len(v) != len(k):
+        raise ValueError("one value per key")
     lib().oracle_stable_sort(KT[kt], _p(k), _p(v), len(k), int(greater))
     return (k, v) if vals is not None else k
 
@@ -154,6 +156,8 @@ def segmented_stable_sort(keys, segments, vals=None, kt="u32", greater=False):
 def stable_sort_radix(keys, vals=None, kt="u32"):
     k = np.ascontiguousarray(keys, dtype=NP[kt]).copy()
     v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    if v is not None and len(v) != len(k):
+        raise ValueError("one value per key")
     lib().oracle_stable_sort_radix(KT[kt], _p(k), _p(v), len(k))
     return (k, v) if vals is not None else k
 
@@ -171,6 +175,8 @@ def merge(a, b, a_vals=None, b_vals=None, kt="u32", greater=False):
     kv = a_vals is not None
     av = np.ascontiguousarray(a_vals, dtype=np.uint32) if kv else None
     bv = np.ascontiguousarray(b_vals, dtype=np.uint32) if kv else None
+    if kv and (len(av) != len(a) or len(bv) != len(b)):
+        raise ValueError("one value per key")
     cv = np.empty(len(c), dtype=np.uint32) if kv else None
     lib().oracle_merge(KT[kt], _p(a), _p(av), len(a), _p(b), _p(bv), len(b), _p(c), _p(cv),
                        int(greater))

@@ -200,3 +200,19 @@ def test_python_surface_rejects_host_tensors():
     import torch
     with pytest.raises(ValueError):
         mgpu_hip.mergesort_keys(torch.zeros(10, dtype=torch.int32))
+
+
+def test_python_merge_rejects_mismatched_shapes():
+    """The C ABI takes raw pointers and the counts of a and b, so the Python
+    surface checks that c and every value array match those counts (and that
+    the key dtypes agree) before anything reaches the device."""
+    import torch
+    k = lambda n, dt=torch.uint32: torch.zeros(n, dtype=dt)
+    with pytest.raises(ValueError, match="b_vals"):
+        mgpu_hip.merge(k(5), k(5), k(3), k(4), k(8), k(8), "less")
+    with pytest.raises(ValueError, match="c_vals"):
+        mgpu_hip.merge(k(5), k(5), k(3), k(3), k(8), k(7), "less")
+    with pytest.raises(ValueError, match="c must hold"):
+        mgpu_hip.merge_keys(k(5), k(3), k(9), "less")
+    with pytest.raises(ValueError, match="one dtype"):
+        mgpu_hip.merge_keys(k(5), k(3, torch.uint64), k(8), "less")

@@ -151,6 +151,40 @@ def test_merge_keys_all_equal_and_greater(comp):
     np.testing.assert_array_equal(to_host(dc), oracle.merge(a, b, greater=greater))
 
 
+@pytest.mark.parametrize("order", ["a_below_b", "b_below_a", "b_inside_a"])
+@pytest.mark.parametrize("comp", ["less", "greater"])
+@pytest.mark.parametrize("kt", ["u32", "u64"])
+def test_merge_kv_disjoint_ranges(order, comp, kt):
+    """Merge-path co-ranks on the edges of the grid: every A key below every B
+    key (and the reverse), and B falling inside one run of equal A keys; the
+    arrays span many tiles, so every partition lies on an edge."""
+    banner("merge disjoint", order, comp, kt)
+    na, nb = 300007, 200003
+    dt = np.uint64 if kt == "u64" else np.uint32
+
+    def draw(n, seed, base):
+        return oracle.generate(n, "u32", seed).astype(dt) % 1000 + dt(base)
+    if order == "a_below_b":
+        a, b = draw(na, 40, 0), draw(nb, 41, 1000)
+    elif order == "b_below_a":
+        a, b = draw(na, 40, 1000), draw(nb, 41, 0)
+    else:  # A: runs of 500, 700 and 900; B: all 700, inside A's run of 700s
+        a = np.concatenate([np.full(na // 3, 500), np.full(na // 3, 700),
+                            np.full(na - 2 * (na // 3), 900)]).astype(dt)
+        b = np.full(nb, 700, dtype=dt)
+    greater = comp == "greater"
+    a = oracle.stable_sort(a, kt=kt, greater=greater)
+    b = oracle.stable_sort(b, kt=kt, greater=greater)
+    av = np.arange(na, dtype=np.uint32)
+    bv = np.arange(na, na + nb, dtype=np.uint32)
+    dc = torch.empty(na + nb, dtype=dev_keys(a).dtype, device="cuda")
+    dcv = torch.empty(na + nb, dtype=torch.uint32, device="cuda")
+    mgpu_hip.merge(dev_keys(a), to_dev(av), dev_keys(b), to_dev(bv), dc, dcv, comp)
+    ec, ecv = oracle.merge(a, b, av, bv, kt, greater=greater)
+    np.testing.assert_array_equal(to_host(dc), ec)
+    np.testing.assert_array_equal(to_host(dcv), ecv)
+
+
 def test_merge_vk_merge_known_answer():
     # examples/vk_merge/vk_merge.cxx:18-27: two sorted arrays of 10000 floats
     # rand() % 100000 (a first, then b from the same rand() stream).

@@ -456,3 +456,15 @@ def test_segmented_stable_sort_vs_numpy_lexsort(kt, greater):
     order = np.lexsort((key_rank, seg_id))
     np.testing.assert_array_equal(v, vals[order])
     np.testing.assert_array_equal(k, keys[order])
+
+
+def test_oracle_rejects_values_of_the_wrong_length():
+    """The oracle passes raw buffers to C: a value array shorter than its
+    keys must be an error, not a read past its end."""
+    a = np.arange(5, dtype=np.uint32)
+    with pytest.raises(ValueError):
+        oracle.merge(a, a, np.arange(5, dtype=np.uint32), np.arange(3, dtype=np.uint32))
+    with pytest.raises(ValueError):
+        oracle.stable_sort(a, np.arange(4, dtype=np.uint32))
+    with pytest.raises(ValueError):
+        oracle.stable_sort_radix(a, np.arange(6, dtype=np.uint32))
