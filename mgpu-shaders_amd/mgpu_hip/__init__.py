@@ -282,30 +282,42 @@ def radix_sort(keys, vals=None, key_type=None, aux=None, stream=None):
 # Scan / reduce (inc/mgpu/vk/scan.hxx:10-115)
 
 _SCAN_T = {torch.int32: "i32", torch.uint32: "u32", torch.float32: "f32", torch.uint64: "u64"}
+_SCAN_BYTES = {"i32": 4, "u32": 4, "f32": 4, "u64": 8}
 
 
-def _scan_type(t, elem_type):
+def _scan_type(t, elem_type, out=None):
+    """Scan element type of t (or elem_type), checked against t's element size
+    and, when given, the output's: the entry points write sizeof(type) bytes
+    per element, so a narrower output would be overrun."""
     if elem_type is not None:
-        return elem_type
-    if t.dtype not in _SCAN_T:
-        raise ValueError("scan supports int32, uint32 and float32")
-    return _SCAN_T[t.dtype]
+        if elem_type not in _SCAN_BYTES:
+            raise ValueError("elem_type must be one of %s" % (tuple(_SCAN_BYTES),))
+        t_type = elem_type
+    elif t.dtype in _SCAN_T:
+        t_type = _SCAN_T[t.dtype]
+    else:
+        raise ValueError("scan supports int32, uint32, float32 and uint64")
+    if t.element_size() != _SCAN_BYTES[t_type]:
+        raise ValueError("input element size does not match %s" % t_type)
+    if out is not None and out.element_size() != t.element_size():
+        raise ValueError("output element size must match the input's")
+    return t_type
 
 
 def scan(data, init=0, elem_type=None, aux=None, stream=None):
     """In-place exclusive plus-scan: data[i] = init + sum(data[:i])."""
-    _device_check(data)
     t = _scan_type(data, elem_type)
+    _device_check(data)
     fn = getattr(load(), "mgpu_scan_exclusive_%s" % t)
     return _run_two_phase(fn, [_ptr(data), data.numel(), _CTYPE[t](init)], "scan", aux, stream)
 
 
 def scan_into(input, output, inclusive=False, init=0, elem_type=None, aux=None, stream=None):
     """Out-of-place (or in-place when output is input) exclusive/inclusive scan."""
-    _device_check(input, output)
     if output.numel() != input.numel():
         raise ValueError("output must match input")
-    t = _scan_type(input, elem_type)
+    t = _scan_type(input, elem_type, output)
+    _device_check(input, output)
     fn = getattr(load(), "mgpu_scan_%s" % t)
     return _run_two_phase(fn, [_ptr(input), _ptr(output), input.numel(),
                                INCLUSIVE if inclusive else EXCLUSIVE, _CTYPE[t](init)],
@@ -316,12 +328,12 @@ def segmented_scan(input, output, segments, inclusive=False, init=0, elem_type=N
                    stream=None):
     """One scan per segment, each from init; segments = int32 device tensor of
     ascending segment starts (element 0 always starts a segment)."""
-    _device_check(input, output, segments)
     if output.numel() != input.numel():
         raise ValueError("output must match input")
     if segments.dtype != torch.int32:
         raise ValueError("segments must be int32")
-    t = _scan_type(input, elem_type)
+    t = _scan_type(input, elem_type, output)
+    _device_check(input, output, segments)
     if t == "u64":
         raise ValueError("segmented scan supports int32, uint32 and float32")
     fn = getattr(load(), "mgpu_segmented_scan_%s" % t)
@@ -332,8 +344,10 @@ def segmented_scan(input, output, segments, inclusive=False, init=0, elem_type=N
 
 def reduce(input, out, init=0, elem_type=None, aux=None, stream=None):
     """out[0] = init + sum(input) (out is a 1-element device tensor)."""
+    if out.numel() < 1:
+        raise ValueError("out must hold one element")
+    t = _scan_type(input, elem_type, out)
     _device_check(input, out)
-    t = _scan_type(input, elem_type)
     fn = getattr(load(), "mgpu_reduce_%s" % t)
     return _run_two_phase(fn, [_ptr(input), input.numel(), _ptr(out), _CTYPE[t](init)],
                           "reduce", aux, stream)

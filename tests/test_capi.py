@@ -216,3 +216,24 @@ def test_python_merge_rejects_mismatched_shapes():
         mgpu_hip.merge_keys(k(5), k(3), k(9), "less")
     with pytest.raises(ValueError, match="one dtype"):
         mgpu_hip.merge_keys(k(5), k(3, torch.uint64), k(8), "less")
+
+
+def test_python_scan_and_reduce_reject_mismatched_element_sizes():
+    """Scans write sizeof(type) bytes per element: an output narrower than the
+    input, an elem_type that does not match the tensor, or an empty reduce
+    output would be overrun, so the Python surface refuses them."""
+    import torch
+    with pytest.raises(ValueError, match="output element size"):
+        mgpu_hip.scan_into(torch.zeros(8, dtype=torch.uint64), torch.zeros(8, dtype=torch.int32))
+    with pytest.raises(ValueError, match="input element size"):
+        mgpu_hip.scan(torch.zeros(8, dtype=torch.int32), elem_type="u64")
+    with pytest.raises(ValueError, match="elem_type"):
+        mgpu_hip.scan(torch.zeros(8, dtype=torch.int32), elem_type="i16")
+    with pytest.raises(ValueError, match="one element"):
+        mgpu_hip.reduce(torch.zeros(8, dtype=torch.int32), torch.zeros(0, dtype=torch.int32))
+    with pytest.raises(ValueError, match="output element size"):
+        mgpu_hip.reduce(torch.zeros(8, dtype=torch.uint64), torch.zeros(1, dtype=torch.int32))
+    with pytest.raises(ValueError, match="output element size"):
+        mgpu_hip.segmented_scan(torch.zeros(8, dtype=torch.float32),
+                                torch.zeros(8, dtype=torch.float64),
+                                torch.zeros(1, dtype=torch.int32))
