@@ -22,19 +22,19 @@ inline mgpu_status_t radix_sort_tuned(void* aux, size_t& aux_size, hipStream_t s
   constexpr bool kv = !std::is_same_v<val_t, empty_t>;
   if constexpr (sizeof(key_t) == 8) {
     if (count <= ((int64_t)1 << 22))
-      return radix_sort<512, 8, 512>(aux, aux_size, stream, keys, vals, count);
-    if constexpr (kv) return radix_sort<256, 24, 512>(aux, aux_size, stream, keys, vals, count);
-    return radix_sort<512, 12, 512>(aux, aux_size, stream, keys, vals, count);
+      return radix_sort<512, 8, 8, 512>(aux, aux_size, stream, keys, vals, count);
+    if constexpr (kv) return radix_sort<256, 24, 8, 512>(aux, aux_size, stream, keys, vals, count);
+    return radix_sort<512, 12, 8, 512>(aux, aux_size, stream, keys, vals, count);
   } else if constexpr (kv) {
     if (count <= ((int64_t)6 << 20))
-      return radix_sort<512, 8, 512>(aux, aux_size, stream, keys, vals, count);
-    return radix_sort<512, 20, 512>(aux, aux_size, stream, keys, vals, count);
+      return radix_sort<512, 8, 8, 512>(aux, aux_size, stream, keys, vals, count);
+    return radix_sort<512, 20, 8, 512>(aux, aux_size, stream, keys, vals, count);
   } else {
     if (count <= ((int64_t)1 << 22))
-      return radix_sort<512, 8, 512>(aux, aux_size, stream, keys, vals, count);
+      return radix_sort<512, 8, 8, 512>(aux, aux_size, stream, keys, vals, count);
     if (count <= ((int64_t)1 << 25))
-      return radix_sort<512, 12, 512>(aux, aux_size, stream, keys, vals, count);
-    return radix_sort<512, 24, 512>(aux, aux_size, stream, keys, vals, count);
+      return radix_sort<512, 12, 8, 512>(aux, aux_size, stream, keys, vals, count);
+    return radix_sort<512, 24, 8, 512>(aux, aux_size, stream, keys, vals, count);
   }
 }
 

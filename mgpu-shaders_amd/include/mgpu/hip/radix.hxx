@@ -618,10 +618,15 @@ inline size_t radix_aux_bytes(int64_t count) {
 }
 
 // Stable LSD radix sort in place; val_t = uint32_t or empty_t.  NT x VT is the
-// scatter's block shape (tile = NT * VT keys); CNT threads count a tile.
-template <int NT = 512, int VT = 24, int CNT = 1024, typename key_t, typename val_t>
+// scatter's block shape (tile = NT * VT keys; NT a multiple of 256); CNT
+// threads count a tile.  num_bits keeps the reference's third template
+// parameter (vk/radix.hxx:195-197, 4 or 8): digits here are always 8 bits
+// wide, which yields the same stable order.
+template <int NT = 512, int VT = 24, int num_bits = 8, int CNT = 512, typename key_t,
+          typename val_t>
 mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
                          val_t* vals, int64_t count) {
+  static_assert(num_bits == 4 || num_bits == 8, "num_bits is 4 or 8, as in the reference");
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   static_assert(!has_values || std::is_same_v<val_t, uint32_t>, "values are uint32");
   constexpr int D = (int)sizeof(key_t);
@@ -717,10 +722,11 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
   return MGPU_OK;
 }
 
-template <int NT = 512, int VT = 24, int CNT = 1024, typename key_t>
+template <int NT = 512, int VT = 24, int num_bits = 8, int CNT = 512, typename key_t>
 mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
                          int64_t count) {
-  return radix_sort<NT, VT, CNT>(aux_data, aux_size, stream, keys, (empty_t*)nullptr, count);
+  return radix_sort<NT, VT, num_bits, CNT>(aux_data, aux_size, stream, keys, (empty_t*)nullptr,
+                                           count);
 }
 
 }  // namespace hip

@@ -1,6 +1,7 @@
 // C++ consumer of the header-template surface (mgpu/hip/*.hxx): user
 // comparator functors through mergesort and merge, launch / transform with
-// device lambdas, and transform_scan with a user functor.  Each check is
+// device lambdas, transform_scan with a user functor, and radix_sort with the
+// reference's template arguments <nt, vt, num_bits>.  Each check is
 // against the std:: algorithm on the host.  Prints "[case] <name>" before
 // each GPU step and "PASS <name>"; exits non-zero on the first mismatch.
 #include <algorithm>
@@ -12,6 +13,7 @@
 #include "mgpu/hip/generate.hxx"
 #include "mgpu/hip/merge.hxx"
 #include "mgpu/hip/mergesort.hxx"
+#include "mgpu/hip/radix.hxx"
 #include "mgpu/hip/scan.hxx"
 #include "mgpu/hip/transform.hxx"
 
@@ -193,6 +195,47 @@ int main() {
     CHECK_HIP(hipFree(x));
     CHECK_HIP(hipFree(y));
     std::printf("PASS transform_scan with a user functor\n");
+  }
+  {
+    // The reference's form radix_sort<nt, vt, num_bits>(aux, aux_size, cmd,
+    // data, count) (vk/radix.hxx:195-197): 4-bit and 8-bit digit requests give
+    // the same stable order; keys only and key-value.
+    banner("radix_sort with the reference's template arguments");
+    uint32_t *keys = nullptr, *vals = nullptr;
+    CHECK_HIP(hipMalloc(&keys, n * 4));
+    CHECK_HIP(hipMalloc(&vals, n * 4));
+    CHECK_OK(generate(keys, n, 4, gen_lowentropy, 41, 0, stream));
+    std::vector<uint32_t> hk = to_host(keys, n);
+    size_t aux_size = 0;
+    CHECK_OK(radix_sort<256, 16, 4>(nullptr, aux_size, stream, keys, n));
+    void* aux = nullptr;
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(radix_sort<256, 16, 4>(aux, aux_size, stream, keys, n));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<uint32_t> sorted = hk;
+    std::stable_sort(sorted.begin(), sorted.end());
+    if (to_host(keys, n) != sorted) { std::printf("FAIL radix_sort<256, 16, 4>\n"); return 1; }
+    CHECK_HIP(hipFree(aux));
+    CHECK_HIP(hipMemcpy(keys, hk.data(), n * 4, hipMemcpyHostToDevice));
+    CHECK_OK(generate(vals, n, 4, gen_iota, 0, 0, stream));
+    aux_size = 0;
+    CHECK_OK(radix_sort<512, 12, 8>(nullptr, aux_size, stream, keys, vals, n));
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(radix_sort<512, 12, 8>(aux, aux_size, stream, keys, vals, n));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return hk[x] < hk[y]; });
+    std::vector<uint32_t> gk = to_host(keys, n), gv = to_host(vals, n);
+    for (int64_t i = 0; i < n; ++i)
+      if (gv[i] != idx[i] || gk[i] != hk[idx[i]]) {
+        std::printf("FAIL radix_sort<512, 12, 8> at %ld\n", (long)i);
+        return 1;
+      }
+    CHECK_HIP(hipFree(aux));
+    CHECK_HIP(hipFree(keys));
+    CHECK_HIP(hipFree(vals));
+    std::printf("PASS radix_sort with the reference's template arguments\n");
   }
   std::printf("ALL PASS\n");
   return 0;
