@@ -22,6 +22,8 @@
 // round trips: fewer tiles per byte is what moves the scan toward the copy
 // bandwidth.
 #pragma once
+#include <functional>
+
 #include "meta.hxx"
 
 namespace mgpu {
@@ -330,6 +332,18 @@ mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, T* data
                    int64_t count, T init = T()) {
   return scan<NT, VT>(aux_data, aux_size, stream, (const T*)data, data, count,
                       MGPU_SCAN_EXCLUSIVE, init);
+}
+
+// The reference's full form, scan<nt, vt>(aux, aux_size, cmd, data, count,
+// init, op) (inc/mgpu/vk/scan.hxx:10-13).  Its thread-local steps add with
+// + whatever op is (common/cta_scan.hxx:150,164), so plus is the only op with
+// a defined result there, and the only one accepted here.
+template <int NT = 256, int VT = 80, typename T, typename op_t>
+mgpu_status_t scan(void* aux_data, size_t& aux_size, hipStream_t stream, T* data,
+                   int64_t count, T init, op_t) {
+  static_assert(std::is_same_v<op_t, std::plus<T>> || std::is_same_v<op_t, std::plus<>>,
+                "the scan operator is addition, as the reference computes it");
+  return scan<NT, VT>(aux_data, aux_size, stream, data, count, init);
 }
 
 ////////////////////////////////////////////////////////////////////////////////

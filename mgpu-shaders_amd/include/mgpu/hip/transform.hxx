@@ -5,7 +5,9 @@
 // transform<nt>(count, cmd, func) calls func(gid) for gid < count.  The same
 // calls here take a hipStream_t in place of the command buffer, accept
 // 64-bit counts and return a status; `func` is any device-callable object
-// (a __device__ lambda or a functor), captured by value.
+// (a __device__ lambda or a functor), captured by value.  The reference's
+// second template parameter, subgroup_size, is kept: -1 (any) or 64, the
+// wave width here.
 #pragma once
 #include "meta.hxx"
 
@@ -24,9 +26,10 @@ __global__ __launch_bounds__(NT) void kernel_transform(int64_t count, func_t fun
 }
 
 // func(tid, cta) for tid < NT, cta < num_blocks.
-template <int NT = 256, typename func_t>
+template <int NT = 256, int subgroup_size = -1, typename func_t>
 mgpu_status_t launch(int num_blocks, hipStream_t stream, func_t func,
                      const char* name = "mgpu_launch") {
+  static_assert(subgroup_size == -1 || subgroup_size == warp_size, "waves are 64 lanes wide");
   if (num_blocks < 0) return MGPU_INVALID_ARG;
   if (num_blocks == 0) return MGPU_OK;
   return hip_status(launch_kernel(name, kernel_launch<NT, func_t>, dim3((unsigned)num_blocks),
@@ -34,9 +37,10 @@ mgpu_status_t launch(int num_blocks, hipStream_t stream, func_t func,
 }
 
 // func(gid) for gid < count.
-template <int NT = 256, typename func_t>
+template <int NT = 256, int subgroup_size = -1, typename func_t>
 mgpu_status_t transform(int64_t count, hipStream_t stream, func_t func,
                         const char* name = "mgpu_transform") {
+  static_assert(subgroup_size == -1 || subgroup_size == warp_size, "waves are 64 lanes wide");
   if (count < 0) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
   const int64_t blocks = div_up(count, (int64_t)NT);

@@ -1,12 +1,14 @@
 // C++ consumer of the header-template surface (mgpu/hip/*.hxx): user
 // comparator functors through mergesort and merge, launch / transform with
-// device lambdas, transform_scan with a user functor, and radix_sort with the
-// reference's template arguments <nt, vt, num_bits>.  Each check is
+// device lambdas, transform_scan with a user functor, and the reference's
+// template and call forms of radix_sort <nt, vt, num_bits>, launch / transform
+// <nt, subgroup_size> and scan(..., init, op).  Each check is
 // against the std:: algorithm on the host.  Prints "[case] <name>" before
 // each GPU step and "PASS <name>"; exits non-zero on the first mismatch.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <numeric>
 #include <vector>
 
@@ -72,8 +74,9 @@ int main() {
     const int blocks = 100;
     CHECK_HIP(hipMalloc(&a, n * sizeof(int64_t)));
     CHECK_HIP(hipMalloc(&b, blocks * 256 * sizeof(int64_t)));
-    CHECK_OK(transform(n, stream, [=] __device__(int64_t i) { a[i] = 3 * i + 1; }));
-    CHECK_OK(launch<256>(blocks, stream, [=] __device__(int tid, int cta) {
+    // The reference's template arguments <nt, subgroup_size>.
+    CHECK_OK(transform<256, -1>(n, stream, [=] __device__(int64_t i) { a[i] = 3 * i + 1; }));
+    CHECK_OK(launch<256, 64>(blocks, stream, [=] __device__(int tid, int cta) {
       b[cta * 256 + tid] = (int64_t)cta * 1000 + tid;
     }));
     CHECK_HIP(hipDeviceSynchronize());
@@ -195,6 +198,30 @@ int main() {
     CHECK_HIP(hipFree(x));
     CHECK_HIP(hipFree(y));
     std::printf("PASS transform_scan with a user functor\n");
+  }
+
+  {
+    // The reference's full in-place form scan<nt, vt>(..., data, count, init, op).
+    banner("scan with an explicit plus operator");
+    int32_t* x = nullptr;
+    CHECK_HIP(hipMalloc(&x, n * 4));
+    CHECK_OK(generate(x, n, 4, gen_i32_0_16, 32, 0, stream));
+    std::vector<int32_t> hx = to_host(x, n);
+    size_t aux_size = 0;
+    CHECK_OK(scan<256, 16>(nullptr, aux_size, stream, x, n, 3, std::plus<int32_t>()));
+    void* aux = nullptr;
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(scan<256, 16>(aux, aux_size, stream, x, n, 3, std::plus<int32_t>()));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::vector<int32_t> hy = to_host(x, n);
+    int32_t run = 3;
+    for (int64_t i = 0; i < n; ++i) {
+      if (hy[i] != run) { std::printf("FAIL scan with plus at %ld\n", (long)i); return 1; }
+      run += hx[i];
+    }
+    CHECK_HIP(hipFree(aux));
+    CHECK_HIP(hipFree(x));
+    std::printf("PASS scan with an explicit plus operator\n");
   }
   {
     // The reference's form radix_sort<nt, vt, num_bits>(aux, aux_size, cmd,
