@@ -618,15 +618,10 @@ inline size_t radix_aux_bytes(int64_t count) {
 }
 
 // Stable LSD radix sort in place; val_t = uint32_t or empty_t.  NT x VT is the
-// scatter's block shape (tile = NT * VT keys; NT a multiple of 256); CNT
-// threads count a tile.  num_bits keeps the reference's third template
-// parameter (vk/radix.hxx:195-197, 4 or 8): digits here are always 8 bits
-// wide, which yields the same stable order.
-template <int NT = 512, int VT = 24, int num_bits = 8, int CNT = 512, typename key_t,
-          typename val_t>
-mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
-                         val_t* vals, int64_t count) {
-  static_assert(num_bits == 4 || num_bits == 8, "num_bits is 4 or 8, as in the reference");
+// scatter's block shape (tile = NT * VT keys); CNT threads count a tile.
+template <int NT, int VT, int CNT, typename key_t, typename val_t>
+mgpu_status_t radix_sort_impl(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
+                              val_t* vals, int64_t count) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   static_assert(!has_values || std::is_same_v<val_t, uint32_t>, "values are uint32");
   constexpr int D = (int)sizeof(key_t);
@@ -720,6 +715,36 @@ mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, k
     std::swap(v_in, v_out);
   }
   return MGPU_OK;
+}
+
+// A tile shape this scatter and count can run: NT covers the 256 digits; the
+// tile splits into the count's 128-bit loads (and, for 64-bit keys, into the
+// digit-byte count's 128 x 16 bytes); the ranking's match table fits in the
+// key buffer; ranks fit 16 bits; the scatter's LDS fits 160 KB.
+template <int NT, int VT, int CNT, typename key_t>
+constexpr bool radix_shape_ok() {
+  constexpr int64_t NV = (int64_t)NT * VT;
+  return NT >= radix_num_bins && NT % radix_num_bins == 0 && NT <= 1024 && VT >= 1 &&
+         CNT >= 64 && CNT <= 1024 && NV % ((16 / (int)sizeof(key_t)) * CNT) == 0 &&
+         (sizeof(key_t) != 8 || NV % 2048 == 0) &&
+         (int64_t)sizeof(key_t) * NV >= 8 * (NT / warp_size) * radix_num_bins &&
+         NV <= 65536 && sizeof(radix_scatter_storage_t<NT, VT, key_t>) <= 160 * 1024;
+}
+
+// The reference's template form radix_sort<nt, vt, num_bits> (vk/radix.hxx:
+// 195-197).  num_bits (4 or 8) is kept for call compatibility: digits here are
+// always 8 bits wide, which yields the same stable order.  nt and vt only tune
+// (the result does not depend on them), so a shape this implementation cannot
+// run, such as the reference's default 128 x 15, runs as 512 x 24.
+template <int NT = 512, int VT = 24, int num_bits = 8, int CNT = 512, typename key_t,
+          typename val_t>
+mgpu_status_t radix_sort(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
+                         val_t* vals, int64_t count) {
+  static_assert(num_bits == 4 || num_bits == 8, "num_bits is 4 or 8, as in the reference");
+  if constexpr (radix_shape_ok<NT, VT, CNT, key_t>())
+    return radix_sort_impl<NT, VT, CNT>(aux_data, aux_size, stream, keys, vals, count);
+  else
+    return radix_sort_impl<512, 24, 512>(aux_data, aux_size, stream, keys, vals, count);
 }
 
 template <int NT = 512, int VT = 24, int num_bits = 8, int CNT = 512, typename key_t>
