@@ -292,10 +292,10 @@ hipError_t scan_launch_cleared(hipStream_t stream, const in_t* input, T* output,
 }
 
 // transform_scan: scan of xform(input[i]).  input/output may alias (in place).
-template <int NT = 256, int VT = 80, typename in_t, typename T, typename xform_t>
-mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
-                             const in_t* input, T* output, int64_t count,
-                             mgpu_scan_type_t type, T init, xform_t xform) {
+template <int NT, int VT, typename in_t, typename T, typename xform_t>
+mgpu_status_t transform_scan_impl(void* aux_data, size_t& aux_size, hipStream_t stream,
+                                  const in_t* input, T* output, int64_t count,
+                                  mgpu_scan_type_t type, T init, xform_t xform) {
   constexpr int NV = NT * VT;
   if (count < 0) return MGPU_INVALID_ARG;
   if (type != MGPU_SCAN_EXCLUSIVE && type != MGPU_SCAN_INCLUSIVE) return MGPU_INVALID_ARG;
@@ -315,6 +315,21 @@ mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t strea
   if (e != hipSuccess) return hip_status(e);
   return hip_status(scan_launch_cleared<NT, VT>(stream, input, output, count, type, init, xform,
                                                 status));
+}
+
+// nt and vt only tune: a shape this kernel cannot run (VT not a multiple of 4,
+// NT not whole waves, such as the reference's default 128 x 7) runs as the
+// default 256 x 80; the result does not depend on the shape.
+template <int NT = 256, int VT = 80, typename in_t, typename T, typename xform_t>
+mgpu_status_t transform_scan(void* aux_data, size_t& aux_size, hipStream_t stream,
+                             const in_t* input, T* output, int64_t count,
+                             mgpu_scan_type_t type, T init, xform_t xform) {
+  if constexpr (VT >= 4 && VT % 4 == 0 && NT >= warp_size && NT % warp_size == 0 && NT <= 1024)
+    return transform_scan_impl<NT, VT>(aux_data, aux_size, stream, input, output, count, type,
+                                       init, xform);
+  else
+    return transform_scan_impl<256, 80>(aux_data, aux_size, stream, input, output, count, type,
+                                        init, xform);
 }
 
 template <int NT = 256, int VT = 80, typename T>

@@ -201,17 +201,18 @@ int main() {
   }
 
   {
-    // The reference's full in-place form scan<nt, vt>(..., data, count, init, op).
+    // The reference's full in-place form scan<nt, vt>(..., data, count, init, op)
+    // with its default shape 128 x 7 (which runs as 256 x 80 here).
     banner("scan with an explicit plus operator");
     int32_t* x = nullptr;
     CHECK_HIP(hipMalloc(&x, n * 4));
     CHECK_OK(generate(x, n, 4, gen_i32_0_16, 32, 0, stream));
     std::vector<int32_t> hx = to_host(x, n);
     size_t aux_size = 0;
-    CHECK_OK(scan<256, 16>(nullptr, aux_size, stream, x, n, 3, std::plus<int32_t>()));
+    CHECK_OK(scan<128, 7>(nullptr, aux_size, stream, x, n, 3, std::plus<int32_t>()));
     void* aux = nullptr;
     CHECK_HIP(hipMalloc(&aux, aux_size));
-    CHECK_OK(scan<256, 16>(aux, aux_size, stream, x, n, 3, std::plus<int32_t>()));
+    CHECK_OK(scan<128, 7>(aux, aux_size, stream, x, n, 3, std::plus<int32_t>()));
     CHECK_HIP(hipDeviceSynchronize());
     std::vector<int32_t> hy = to_host(x, n);
     int32_t run = 3;
