@@ -122,7 +122,7 @@ int main() {
   }
 
   {
-    banner("merge with a user comparator");
+    banner("merge with a user comparator");  // at the reference's default shape 128 x 7
     const int64_t na = n, nb = n / 3 + 5;
     std::vector<float> ha(na), hb(nb);
     float *a = nullptr, *b = nullptr, *c = nullptr;
@@ -148,11 +148,11 @@ int main() {
     CHECK_OK(transform(na, stream, [=] __device__(int64_t i) { av[i] = (uint32_t)i; }));
     CHECK_OK(transform(nb, stream, [=] __device__(int64_t i) { bv[i] = (uint32_t)(na + i); }));
     size_t aux_size = 0;
-    CHECK_OK(merge<256, 15>(nullptr, aux_size, stream, (const float*)a, (const uint32_t*)av, na,
+    CHECK_OK(merge<128, 7>(nullptr, aux_size, stream, (const float*)a, (const uint32_t*)av, na,
                             (const float*)b, (const uint32_t*)bv, nb, c, cv, abs_less_t()));
     void* aux = nullptr;
     CHECK_HIP(hipMalloc(&aux, aux_size));
-    CHECK_OK(merge<256, 15>(aux, aux_size, stream, (const float*)a, (const uint32_t*)av, na,
+    CHECK_OK(merge<128, 7>(aux, aux_size, stream, (const float*)a, (const uint32_t*)av, na,
                             (const float*)b, (const uint32_t*)bv, nb, c, cv, abs_less_t()));
     CHECK_HIP(hipDeviceSynchronize());
     std::vector<std::pair<float, uint32_t>> pa(na), pb(nb), pc(na + nb);
