@@ -248,10 +248,9 @@ inline size_t mergesort_aux_bytes(int64_t count) {
   return need;
 }
 
-template <int NT = 256, int VT = 15, bool sort_indices = false, typename key_t, typename val_t,
-          typename comp_t = less_t<key_t>>
-mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
-                           val_t* vals, int64_t count, comp_t comp = comp_t()) {
+template <int NT, int VT, bool sort_indices, typename key_t, typename val_t, typename comp_t>
+mgpu_status_t mergesort_kv_impl(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
+                                val_t* vals, int64_t count, comp_t comp) {
   static_assert(!sort_indices || std::is_same_v<val_t, int32_t>, "indices are int32");
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   constexpr int NV = NT * VT;
@@ -305,6 +304,23 @@ mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream,
     std::swap(v_in, v_out);
   }
   return MGPU_OK;
+}
+
+// nt and vt only tune: a shape the blocksort cannot run (NT not a power of two
+// up to 1024, VT beyond the 31-slot out-of-range mask, LDS beyond 160 KB) runs
+// as the default 256 x 15; the sorted result does not depend on the shape.
+template <int NT = 256, int VT = 15, bool sort_indices = false, typename key_t, typename val_t,
+          typename comp_t = less_t<key_t>>
+mgpu_status_t mergesort_kv(void* aux_data, size_t& aux_size, hipStream_t stream, key_t* keys,
+                           val_t* vals, int64_t count, comp_t comp = comp_t()) {
+  if constexpr (NT >= warp_size && NT <= 1024 && is_pow2(NT) && VT >= 1 && VT < 32 &&
+                sizeof(blocksort_storage_t<NT, VT, key_t, val_t>) <= 160 * 1024 &&
+                sizeof(merge_storage_t<NT, VT, key_t, val_t>) <= 160 * 1024)
+    return mergesort_kv_impl<NT, VT, sort_indices>(aux_data, aux_size, stream, keys, vals, count,
+                                                   comp);
+  else
+    return mergesort_kv_impl<256, 15, sort_indices>(aux_data, aux_size, stream, keys, vals, count,
+                                                    comp);
 }
 
 template <int NT = 256, int VT = 15, typename key_t, typename comp_t = less_t<key_t>>
