@@ -1,7 +1,7 @@
 // C++ consumer of the header-template surface (mgpu/hip/*.hxx): user
 // comparator functors through mergesort and merge, launch / transform with
-// device lambdas, transform_scan with a user functor, and the reference's
-// template and call forms of radix_sort <nt, vt, num_bits>, launch / transform
+// device lambdas, transform_scan with a user functor, std::greater as a
+// comparator, and the reference's template and call forms of radix_sort <nt, vt, num_bits>, launch / transform
 // <nt, subgroup_size> and scan(..., init, op).  Each check is
 // against the std:: algorithm on the host.  Prints "[case] <name>" before
 // each GPU step and "PASS <name>"; exits non-zero on the first mismatch.
@@ -224,6 +224,27 @@ int main() {
     CHECK_HIP(hipFree(x));
     std::printf("PASS scan with an explicit plus operator\n");
   }
+  {
+    // The reference's default comparator type, std::less / std::greater
+    // (constexpr, so host-and-device in HIP), passed unchanged.
+    banner("mergesort_keys with std::greater");
+    uint32_t* keys = nullptr;
+    CHECK_HIP(hipMalloc(&keys, n * 4));
+    CHECK_OK(generate(keys, n, 4, gen_lowentropy, 51, 0, stream));
+    std::vector<uint32_t> hk = to_host(keys, n);
+    size_t aux_size = 0;
+    CHECK_OK(mergesort_keys<128, 7>(nullptr, aux_size, stream, keys, n, std::greater<uint32_t>()));
+    void* aux = nullptr;
+    CHECK_HIP(hipMalloc(&aux, aux_size));
+    CHECK_OK(mergesort_keys<128, 7>(aux, aux_size, stream, keys, n, std::greater<uint32_t>()));
+    CHECK_HIP(hipDeviceSynchronize());
+    std::stable_sort(hk.begin(), hk.end(), std::greater<uint32_t>());
+    if (to_host(keys, n) != hk) { std::printf("FAIL mergesort_keys with std::greater\n"); return 1; }
+    CHECK_HIP(hipFree(aux));
+    CHECK_HIP(hipFree(keys));
+    std::printf("PASS mergesort_keys with std::greater\n");
+  }
+
   {
     // The reference's form radix_sort<nt, vt, num_bits>(aux, aux_size, cmd,
     // data, count) (vk/radix.hxx:195-197): 4-bit and 8-bit digit requests give
