@@ -1,9 +1,10 @@
 // C++ consumer of the header-template surface (mgpu/hip/*.hxx): user
 // comparator functors through mergesort and merge, launch / transform with
 // device lambdas, transform_scan with a user functor, std::greater as a
-// comparator, and the reference's template and call forms of radix_sort <nt, vt, num_bits>, launch / transform
-// <nt, subgroup_size> and scan(..., init, op).  Each check is
-// against the std:: algorithm on the host.  Prints "[case] <name>" before
+// comparator, and the reference's template and call forms: radix_sort
+// <nt, vt, num_bits>, launch / transform <nt, subgroup_size>, scan(..., init,
+// op) and its default shapes.  Each check is against the std:: algorithm on
+// the host.  Prints "[case] <name>" before
 // each GPU step and "PASS <name>"; exits non-zero on the first mismatch.
 #include <algorithm>
 #include <cmath>
