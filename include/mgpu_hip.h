@@ -66,13 +66,22 @@ const char* mgpu_status_string(mgpu_status_t status);
 int mgpu_last_hip_error(void);
 /* Library version: major*10000 + minor*100 + patch. */
 int mgpu_version(void);
+/* Creates the library's per-device state on the current device (the
+ * look-back fallback counter: 256 bytes, allocated and zeroed once).
+ * Optional: any entry point creates what it needs on first use, except
+ * while its stream is being captured into a graph, when it runs without the
+ * counter (only the fallback count is lost).  Stands in for the reference's
+ * context_t setup (src/vk/context.cxx:20-130), of which nothing else is
+ * needed here.                                                             */
+mgpu_status_t mgpu_init(void);
 
 /* ------------------------------------------------------------------------ */
 /* Mergesort: stable, in place.  Replaces vk::mergesort_keys / mergesort_kv /
  * mergesort_indices (inc/mgpu/vk/mergesort.hxx:86-93, 10-84, 95-103).
  * aux: ping-pong keys (+ values) and merge-path partitions.
- * Key types: u32, i32, u64, f32.  Values: u32.  indices: vals[i] := i before
- * the sort (inc/mgpu/common/kernel_mergesort.hxx:43-49).  count < 2^31.     */
+ * Key types: u32, i32, u64, f32, i64, f64.  Values: u32.  indices:
+ * vals[i] := i before the sort (inc/mgpu/common/kernel_mergesort.hxx:43-49).
+ * count < 2^31.                                                             */
 #define MGPU_DECL_MERGESORT(K, KT)                                             \
   mgpu_status_t mgpu_mergesort_keys_##K(void* aux, size_t* aux_size,           \
       mgpu_stream_t stream, KT* keys, int64_t count, mgpu_comp_t comp);        \
@@ -312,7 +321,10 @@ mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int 
  * pass kernels).  Scan and radix tiles wait for a predecessor's published
  * word at most `limit` polls, then compute that predecessor's contribution
  * from its input, so progress never depends on workgroup dispatch order.
- * Tests set a tiny limit to force that path; results are identical.
+ * Tests set a tiny limit to force that path.  Integer results are
+ * identical either way; a float32 scan sums a fallback tile element by
+ * element in float64, so its results may differ in the last bits, within
+ * the 2e-6 bound stated for the float32 scans.
  * mgpu_lookback_fallbacks returns how often it was taken on the current
  * device (synchronising the device) and optionally resets the count.       */
 /* Digit-pass structure of the radix sort (process-wide): 0 = automatic

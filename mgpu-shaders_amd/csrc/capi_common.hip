@@ -24,6 +24,8 @@ int mgpu_last_hip_error(void) { return last_hip_error_slot(); }
 
 int mgpu_version(void) { return 100; }
 
+mgpu_status_t mgpu_init(void) { return init(); }
+
 void mgpu_timing_enable(int enable) {
   timing_registry_t& tr = timing_registry();
   std::lock_guard<std::mutex> lock(tr.mutex);

@@ -211,18 +211,48 @@ inline lookback_settings_t& lookback_settings() {
   static lookback_settings_t s;
   return s;
 }
-inline lookback_config_t lookback_config() {
+// The look-back settings for a launch on `stream`, on the current device.
+// The fallback counter of a device is allocated and zeroed on its first use
+// (or by init()); a launch recorded while `stream` is being captured into a
+// graph never allocates: if the counter does not exist yet, that launch runs
+// without one (it only loses the count), so a caller may begin capturing
+// before any other call into the library.
+inline lookback_config_t lookback_config(hipStream_t stream = nullptr) {
   lookback_settings_t& s = lookback_settings();
   std::lock_guard<std::mutex> lock(s.mutex);
   int dev = 0;
   (void)hipGetDevice(&dev);
   if ((int)s.counters.size() <= dev) s.counters.resize(dev + 1, nullptr);
   if (!s.counters[dev]) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (stream && hipStreamIsCapturing(stream, &cs) == hipSuccess &&
+        cs != hipStreamCaptureStatusNone)
+      return lookback_config_t{s.spin_limit, nullptr};
     uint32_t* p = nullptr;
     if (hipMalloc(&p, 256) == hipSuccess && hipMemset(p, 0, 256) == hipSuccess)
       s.counters[dev] = p;
   }
   return lookback_config_t{s.spin_limit, s.counters[dev]};
+}
+
+// The look-back settings without creating anything: the current device's
+// fallback counter if it exists already, else none (the launch then only
+// loses the count).  For callers that must not allocate.
+inline lookback_config_t lookback_config_existing() {
+  lookback_settings_t& s = lookback_settings();
+  std::lock_guard<std::mutex> lock(s.mutex);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  uint32_t* c = dev < (int)s.counters.size() ? s.counters[dev] : nullptr;
+  return lookback_config_t{s.spin_limit, c};
+}
+
+// Creates the library's per-device state on the current device (the
+// look-back fallback counter).  Optional: every entry point creates it on
+// first use outside a graph capture; call it once before capturing if the
+// captured launches should count look-back fallbacks.
+inline mgpu_status_t init() {
+  return lookback_config().fallbacks ? MGPU_OK : MGPU_HIP_ERROR;
 }
 
 ////////////////////////////////////////////////////////////////////////////////
@@ -293,7 +323,10 @@ __device__ inline uint64_t match_digit8(uint32_t digit, uint64_t active) {
 }
 
 // Relaxed agent-scope atomics for the look-back state words.  Each word holds
-// its payload and status together, so no ordering with other memory is needed.
+// its payload and status together, so reading a word needs no ordering with
+// other memory.  The one place where order matters is the in-place scan's
+// input fallback (a tile summing a predecessor's input while that tile may
+// overwrite it with output): see status_store_complete / acquire_agent.
 template <typename T>
 __device__ inline T atomic_load_agent(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -306,6 +339,22 @@ template <typename T>
 __device__ inline T atomic_add_agent(T* p, T v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Owner side of the in-place look-back: a tile's status word (an agent-scope
+// store, written through to the point of coherence) must be visible before
+// any later store of the tile's output.  s_waitcnt(0) holds the wave until
+// the store is acknowledged, and the signal fence keeps the compiler from
+// moving later stores above it.  (An agent-scope release fence orders them
+// too, but on gfx950 it writes back the L2, once per tile.)
+__device__ inline void status_store_complete() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// Reader side: the loads of a predecessor's input complete, and no stale
+// cache line serves a later load, before its status word is read again.  If
+// any element read was already the predecessor's output, its status store
+// completed before that output was written, so the re-read sees it.
+__device__ inline void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // Zero-fill of 32-bit words (used instead of hipMemsetAsync so the
 // schedule stays capturable in a graph and replays correctly).

@@ -653,7 +653,7 @@ mgpu_status_t radix_sort_impl(void* aux_data, size_t& aux_size, hipStream_t stre
     uint32_t* status = advance_pointer<uint32_t>(p, num_counts * D);
     uint32_t* all_counts = advance_pointer<uint32_t>(p, (int64_t)D * radix_num_bins);
     uint32_t* bases = advance_pointer<uint32_t>(p, (int64_t)D * radix_num_bins);
-    const lookback_config_t lb = lookback_config();
+    const lookback_config_t lb = lookback_config(stream);
     e = fill_u32(status, num_counts * D, 0u, stream);
     if (e == hipSuccess) e = fill_u32(all_counts, (int64_t)D * radix_num_bins, 0u, stream);
     // At least 8 vectors per thread; at most 4096 blocks.

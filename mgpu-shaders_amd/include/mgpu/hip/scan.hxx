@@ -100,6 +100,7 @@ __device__ inline carry_t scan_lookback(const uint64_t* status, int64_t tile, ca
       const int kb = __builtin_ctzll(bad_mask & range);
       const int64_t jj = j - kb;
       carry_t agg = wave_tile_sum<carry_t, acc_t, in_t, T>(input, jj * NV, NV, xform);
+      acquire_agent();
       const uint64_t again = atomic_load_agent(status + jj);
       carry_t v = lane < kb ? unpack_status<carry_t>(w) : carry_t(0);
       excl += wave_reduce_all(v);
@@ -214,8 +215,8 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
       if (lane == 0) {
         atomic_store_agent(status + tile, pack_status(aggregate, status_aggregate));
         // The aggregate must be visible before this tile's output (possibly
-        // over its own input) can be: wait for the store to complete.
-        __builtin_amdgcn_s_waitcnt(0);
+        // over its own input) can be.
+        status_store_complete();
       }
       carry = scan_lookback<carry_t, acc_t, NV, in_t, T>(status, tile, (carry_t)(acc_t)init,
                                                          input, xform, lb);
@@ -277,10 +278,10 @@ __global__ __launch_bounds__(NT) void kernel_scan_lookback(
 // the start of its aux).  Arguments are not checked again.
 template <int NT, int VT, typename in_t, typename T, typename xform_t>
 hipError_t scan_launch_cleared(hipStream_t stream, const in_t* input, T* output, int64_t count,
-                               mgpu_scan_type_t type, T init, xform_t xform, uint64_t* status) {
+                               mgpu_scan_type_t type, T init, xform_t xform, uint64_t* status,
+                               lookback_config_t lb) {
   constexpr int NV = NT * VT;
   const int64_t num_tiles = div_up(count, (int64_t)NV);
-  const lookback_config_t lb = lookback_config();
   int vec_ok = sizeof(in_t) == 4 && 0 == ((uintptr_t)input & 15) && 0 == ((uintptr_t)output & 15);
   if (type == MGPU_SCAN_INCLUSIVE)
     return launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, true, in_t, T, xform_t>,
@@ -289,6 +290,12 @@ hipError_t scan_launch_cleared(hipStream_t stream, const in_t* input, T* output,
   return launch_kernel("mgpu_scan_lookback", kernel_scan_lookback<NT, VT, false, in_t, T, xform_t>,
                        dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
                        num_tiles, init, status, lb, vec_ok, xform);
+}
+template <int NT, int VT, typename in_t, typename T, typename xform_t>
+hipError_t scan_launch_cleared(hipStream_t stream, const in_t* input, T* output, int64_t count,
+                               mgpu_scan_type_t type, T init, xform_t xform, uint64_t* status) {
+  return scan_launch_cleared<NT, VT>(stream, input, output, count, type, init, xform, status,
+                                     lookback_config(stream));
 }
 
 // transform_scan: scan of xform(input[i]).  input/output may alias (in place).
@@ -550,6 +557,7 @@ __device__ inline uint64_t scan64_lookback(const uint64_t* status, int64_t tile,
       for (int e = lane; e < NV; e += warp_size) agg += (uint64_t)input[jj * NV + e];
       agg = wave_reduce_all(agg);
       uint64_t again;
+      acquire_agent();
       const uint32_t f_again = scan64_read(status, jj, again);
       excl += wave_reduce_all(lane < kb ? v : (uint64_t)0);
       spins = 0;
@@ -636,8 +644,8 @@ __global__ __launch_bounds__(NT) void kernel_scan64_lookback(const T* input, T* 
       if (lane == 0) {
         scan64_publish(status, tile, aggregate, status_aggregate);
         // The aggregate must be visible before this tile's output (possibly
-        // over its own input) can be: wait for the stores to complete.
-        __builtin_amdgcn_s_waitcnt(0);
+        // over its own input) can be.
+        status_store_complete();
       }
       carry = scan64_lookback<NV>(status, tile, (uint64_t)init, input, lb);
     }
@@ -709,7 +717,7 @@ mgpu_status_t scan64(void* aux_data, size_t& aux_size, hipStream_t stream, const
   uint64_t* status = (uint64_t*)aux_data;
   hipError_t e = fill_u32((uint32_t*)status, 4 * num_tiles, 0u, stream);
   if (e != hipSuccess) return hip_status(e);
-  const lookback_config_t lb = lookback_config();
+  const lookback_config_t lb = lookback_config(stream);
   if (type == MGPU_SCAN_INCLUSIVE)
     e = launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, true, T>,
                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,

@@ -141,6 +141,7 @@ __device__ inline carry_t segscan_lookback(const uint64_t* status, int64_t tile,
       bool head;
       carry_t tail = segscan_tile_tail<carry_t, acc_t, NV, in_t, T>(input, segments, tile_first,
                                                                     jj, xform, head);
+      acquire_agent();
       const uint64_t again = atomic_load_agent(status + jj);
       carry_t v = lane < kb ? unpack_status<carry_t>(w) : carry_t(0);
       excl += wave_reduce_all(v);
@@ -292,8 +293,8 @@ __global__ __launch_bounds__(NT) void kernel_segscan(
                                                    status_inclusive)
                                      : pack_status(aggregate.v, status_aggregate));
         // The word must be visible before this tile's output (possibly over
-        // its own input) can be: wait for the store to complete.
-        __builtin_amdgcn_s_waitcnt(0);
+        // its own input) can be.
+        status_store_complete();
       }
       if (!head_tile || !(s_heads[0] & 1u))
         carry = segscan_lookback<carry_t, acc_t, NV, in_t, T>(
@@ -388,7 +389,7 @@ mgpu_status_t segmented_transform_scan(void* aux_data, size_t& aux_size, hipStre
   void* p = aux_data;
   uint64_t* status = advance_pointer<uint64_t>(p, num_tiles);
   int32_t* tile_first = advance_pointer<int32_t>(p, num_tiles + 1);
-  const lookback_config_t lb = lookback_config();
+  const lookback_config_t lb = lookback_config(stream);
   hipError_t e = fill_u32((uint32_t*)status, 2 * num_tiles, 0u, stream);
   if (e != hipSuccess) return hip_status(e);
   if (num_segments > 0) {

@@ -69,10 +69,14 @@ inline segsort_tuning_t segsort_default_tuning() {
 
 struct segsort_bucket_caps_t { int cap[segsort_max_buckets]; int num_buckets; };
 
+// Segment i is [begin, end) with both ends clamped to [0, count] and
+// end >= begin: a start at or beyond count gives an empty segment, as in the
+// segmented scan, and no start can place a segment outside the arrays.
 __device__ inline void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
                                       int64_t i, int64_t& begin, int64_t& end) {
-  begin = segments[i];
-  end = i + 1 < num_segments ? (int64_t)segments[i + 1] : count;
+  begin = min_<int64_t>(max_<int64_t>(segments[i], 0), count);
+  const int64_t next = i + 1 < num_segments ? (int64_t)segments[i + 1] : count;
+  end = min_<int64_t>(max_<int64_t>(next, begin), count);
 }
 
 // 1. flags[b * S + i] for b in [0, num_buckets] (num_buckets = large), and
@@ -80,7 +84,7 @@ __device__ inline void segment_bounds(const int32_t* segments, int64_t num_segme
 template <int NT>
 __global__ __launch_bounds__(NT) void kernel_segsort_classify(
     const int32_t* segments, int64_t num_segments, int64_t count, segsort_bucket_caps_t caps,
-    uint32_t* flags, uint32_t* large_tiles) {
+    uint32_t* flags, uint64_t* large_tiles) {
   const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (i >= num_segments) return;
   int64_t b0, b1;
@@ -94,17 +98,22 @@ __global__ __launch_bounds__(NT) void kernel_segsort_classify(
   }
   for (int b = 0; b <= caps.num_buckets; ++b)
     flags[b * num_segments + i] = bucket == b ? 1u : 0u;
-  large_tiles[i] = bucket == caps.num_buckets ? (uint32_t)div_up(len, (int64_t)segsort_radix_nv) : 0u;
+  large_tiles[i] = bucket == caps.num_buckets ? (uint64_t)div_up(len, (int64_t)segsort_radix_nv) : 0u;
 }
 
 // 2. list[pos[k]] = segment of flag k; bounds[b] = {start, end} of bucket b's
 //    list; large_tasks[tile_pos[i] + t] = (i, t) for every tile t of a large
 //    segment i; *num_tasks = total large tiles.
+//    Segments whose starts ascend (the contract) have at most max_tiles tiles
+//    in total.  Starts that do not ascend make segments overlap and can
+//    exceed it: a segment whose tiles do not all fit below max_tiles is then
+//    left unsorted (its slots below max_tiles get the task (-1, 0), which the
+//    radix kernels skip), so no write leaves the aux arrays.
 template <int NT>
 __global__ __launch_bounds__(NT) void kernel_segsort_compact(
     const uint32_t* flags, const uint32_t* pos, int64_t num_segments, int num_buckets,
-    int32_t* list, int32_t* bounds, const uint32_t* large_tiles, const uint32_t* tile_pos,
-    int2* large_tasks, int32_t* num_tasks) {
+    int32_t* list, int32_t* bounds, const uint64_t* large_tiles, const uint64_t* tile_pos,
+    int2* large_tasks, int32_t* num_tasks, int64_t max_tiles) {
   const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
   const int64_t total = (int64_t)(num_buckets + 1) * num_segments;
   if (k < total && flags[k]) list[pos[k]] = (int32_t)(k % num_segments);
@@ -115,9 +124,11 @@ __global__ __launch_bounds__(NT) void kernel_segsort_compact(
                                          : (int32_t)pos[(int64_t)(b + 1) * num_segments];
   }
   if (k < num_segments) {
-    const uint32_t n = large_tiles[k];
-    for (uint32_t t = 0; t < n; ++t) large_tasks[tile_pos[k] + t] = make_int2((int)k, (int)t);
-    if (k == num_segments - 1) *num_tasks = (int32_t)(tile_pos[k] + n);
+    const uint64_t n = large_tiles[k], first = tile_pos[k];
+    const bool fits = first + n <= (uint64_t)max_tiles;
+    for (uint64_t t = 0; t < n && first + t < (uint64_t)max_tiles; ++t)
+      large_tasks[first + t] = fits ? make_int2((int)k, (int)t) : make_int2(-1, 0);
+    if (k == num_segments - 1) *num_tasks = (int32_t)min_<uint64_t>(first + n, (uint64_t)max_tiles);
   }
 }
 
@@ -158,8 +169,8 @@ struct segsort_bits_t {
 template <typename key_t, bool descending>
 __global__ __launch_bounds__(segsort_radix_cnt) void kernel_segsort_radix_count(
     const key_t* keys, int64_t count, const int32_t* segments, int64_t num_segments,
-    const int2* tasks, const int32_t* num_tasks, const uint32_t* large_tiles,
-    const uint32_t* tile_pos, int shift, uint32_t* counts, uint32_t* clear, int64_t clear_words) {
+    const int2* tasks, const int32_t* num_tasks, const uint64_t* large_tiles,
+    const uint64_t* tile_pos, int shift, uint32_t* counts, uint32_t* clear, int64_t clear_words) {
   __shared__ uint32_t h[radix_num_bins * 16];
   // Also zero the status words of the counts scan that follows.
   for (int64_t i = (int64_t)blockIdx.x * segsort_radix_cnt + threadIdx.x; i < clear_words;
@@ -168,11 +179,12 @@ __global__ __launch_bounds__(segsort_radix_cnt) void kernel_segsort_radix_count(
   const int32_t n = *num_tasks;
   for (int64_t e = blockIdx.x; e < n; e += gridDim.x) {
     const int2 task = tasks[e];
+    if (task.x < 0) continue;  // a segment left unsorted (see kernel_segsort_compact)
     int64_t b0, b1;
     segment_bounds(segments, num_segments, count, task.x, b0, b1);
     const int64_t base = b0 + (int64_t)task.y * segsort_radix_nv;
     const int tile_count = (int)min_<int64_t>(segsort_radix_nv, b1 - base);
-    const uint32_t tiles = large_tiles[task.x];
+    const uint32_t tiles = (uint32_t)large_tiles[task.x];
     cta_radix_count<segsort_radix_cnt, segsort_radix_nv>(
         keys + base, tile_count, shift, segsort_bits_t<key_t, descending>(),
         counts + 256ull * tile_pos[task.x] + task.y, tiles, h);
@@ -185,18 +197,19 @@ template <typename key_t, bool has_values, bool descending>
 __global__ __launch_bounds__(segsort_radix_nt) void kernel_segsort_radix_scatter(
     const key_t* keys_in, key_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
     int64_t count, const int32_t* segments, int64_t num_segments, const int2* tasks,
-    const int32_t* num_tasks, const uint32_t* large_tiles, const uint32_t* tile_pos, int shift,
+    const int32_t* num_tasks, const uint64_t* large_tiles, const uint64_t* tile_pos, int shift,
     const uint32_t* offsets) {
   __shared__ radix_scatter_storage_t<segsort_radix_nt, segsort_radix_vt, key_t> s;
   const int tid = threadIdx.x;
   const int32_t n = *num_tasks;
   for (int64_t e = blockIdx.x; e < n; e += gridDim.x) {
     const int2 task = tasks[e];
+    if (task.x < 0) continue;  // uniform over the block: one task per block
     int64_t b0, b1;
     segment_bounds(segments, num_segments, count, task.x, b0, b1);
     const int64_t base = b0 + (int64_t)task.y * segsort_radix_nv;
     const int tile_count = (int)min_<int64_t>(segsort_radix_nv, b1 - base);
-    const uint32_t tiles = large_tiles[task.x];
+    const uint32_t tiles = (uint32_t)large_tiles[task.x];
     const uint64_t cbase = 256ull * tile_pos[task.x];
     // Position of the segment's first key in the concatenation of the large
     // segments (digit 0, tile 0 of this segment), mapped back to b0.
@@ -234,7 +247,7 @@ inline size_t segmented_sort_aux_bytes(int64_t count, int64_t num_segments,
   need += aux_bytes<uint32_t>(B * S) * 2;         // flags, pos
   need += aux_bytes<int32_t>(B * S);              // lists
   need += aux_bytes<int32_t>(2 * B);              // bounds
-  need += aux_bytes<uint32_t>(S) * 2;             // large tiles, tile_pos
+  need += aux_bytes<uint64_t>(S) * 2;             // large tiles, tile_pos
   need += aux_bytes<int2>(max_tiles);             // tasks
   need += aux_bytes<int32_t>(1);                  // number of tasks
   need += aux_bytes<uint32_t>(max_counts);        // counts / offsets
@@ -242,8 +255,11 @@ inline size_t segmented_sort_aux_bytes(int64_t count, int64_t num_segments,
   if (has_values) need += aux_bytes<uint32_t>(count);
   size_t scan_need = 0;
   scan<256, 16>(nullptr, scan_need, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                max(max(B * S, S), max_counts), MGPU_SCAN_EXCLUSIVE, 0u);
-  return need + scan_need;
+                max(B * S, max_counts), MGPU_SCAN_EXCLUSIVE, 0u);
+  size_t scan64_need = 0;
+  scan64<uint64_t>(nullptr, scan64_need, nullptr, (const uint64_t*)nullptr, (uint64_t*)nullptr, S,
+                   MGPU_SCAN_EXCLUSIVE, (uint64_t)0);
+  return need + max(scan_need, scan64_need);
 }
 
 template <int SHAPE, typename key_t, typename val_t, typename comp_t>
@@ -311,8 +327,8 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   uint32_t* pos = advance_pointer<uint32_t>(p, B * S);
   int32_t* list = advance_pointer<int32_t>(p, B * S);
   int32_t* bounds = advance_pointer<int32_t>(p, 2 * B);
-  uint32_t* large_tiles = advance_pointer<uint32_t>(p, S);
-  uint32_t* tile_pos = advance_pointer<uint32_t>(p, S);
+  uint64_t* large_tiles = advance_pointer<uint64_t>(p, S);
+  uint64_t* tile_pos = advance_pointer<uint64_t>(p, S);
   int2* tasks = advance_pointer<int2>(p, max_tiles);
   int32_t* num_tasks = advance_pointer<int32_t>(p, 1);
   uint32_t* counts = advance_pointer<uint32_t>(p, max_counts);
@@ -331,13 +347,16 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   mgpu_status_t st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)flags, pos, B * S,
                                    MGPU_SCAN_EXCLUSIVE, 0u);
   if (st != MGPU_OK) return st;
-  st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)large_tiles, tile_pos, S,
-                     MGPU_SCAN_EXCLUSIVE, 0u);
+  // 64-bit tile positions: their sum cannot wrap even when overlapping
+  // segments (starts that do not ascend) claim more tiles than exist.
+  st = scan64<uint64_t>(p, scan_cap, stream, (const uint64_t*)large_tiles, tile_pos, S,
+                        MGPU_SCAN_EXCLUSIVE, (uint64_t)0);
   if (st != MGPU_OK) return st;
   e = launch_kernel("mgpu_segsort_compact", kernel_segsort_compact<256>,
                     dim3((unsigned)div_up(B * S, (int64_t)256)), dim3(256), 0, stream,
                     (const uint32_t*)flags, (const uint32_t*)pos, S, NB, list, bounds,
-                    (const uint32_t*)large_tiles, (const uint32_t*)tile_pos, tasks, num_tasks);
+                    (const uint64_t*)large_tiles, (const uint64_t*)tile_pos, tasks, num_tasks,
+                    max_tiles);
   if (e != hipSuccess) return hip_status(e);
 
   // Blocksort buckets.
@@ -365,7 +384,7 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
     e = launch_kernel("mgpu_segsort_radix_count", kernel_segsort_radix_count<key_t, descending>,
                       dim3(grid), dim3(segsort_radix_cnt), 0, stream, (const key_t*)k_in, count,
                       segments, S, (const int2*)tasks, (const int32_t*)num_tasks,
-                      (const uint32_t*)large_tiles, (const uint32_t*)tile_pos, shift, counts,
+                      (const uint64_t*)large_tiles, (const uint64_t*)tile_pos, shift, counts,
                       (uint32_t*)scan_status, scan_words);
     if (e != hipSuccess) return hip_status(e);
     e = scan_launch_cleared<256, 16>(stream, (const uint32_t*)counts, counts, max_counts,
@@ -375,8 +394,8 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
                       kernel_segsort_radix_scatter<key_t, has_values, descending>, dim3(grid),
                       dim3(segsort_radix_nt), 0, stream, (const key_t*)k_in, k_out,
                       (const uint32_t*)v_in, v_out, count, segments, S, (const int2*)tasks,
-                      (const int32_t*)num_tasks, (const uint32_t*)large_tiles,
-                      (const uint32_t*)tile_pos, shift, (const uint32_t*)counts);
+                      (const int32_t*)num_tasks, (const uint64_t*)large_tiles,
+                      (const uint64_t*)tile_pos, shift, (const uint32_t*)counts);
     if (e != hipSuccess) return hip_status(e);
     std::swap(k_in, k_out);
     std::swap(v_in, v_out);

@@ -178,12 +178,25 @@ def _device_check(*ts):
 
 
 def _run_two_phase(fn, args, where, aux=None, stream=None):
-    """Sizing call (aux=NULL) then the real call; allocates aux unless given."""
+    """Sizing call (aux=NULL) then the real call; allocates aux unless given.
+
+    The call is enqueued on `stream` (default: torch's current stream).  An
+    aux tensor is allocated on that stream, and when `stream` is not torch's
+    current stream the aux is also marked as used by it (record_stream), so
+    torch's caching allocator does not hand its memory to another allocation
+    while the library may still be writing it, whether or not the caller
+    keeps the returned aux."""
     lib = load()
     size = ctypes.c_size_t(0)
     _check(fn(None, ctypes.byref(size), _stream(stream), *args), where + " (sizing)")
     if aux is None:
-        aux = torch.empty(max(int(size.value), 1), dtype=torch.uint8, device="cuda")
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                aux = torch.empty(max(int(size.value), 1), dtype=torch.uint8, device="cuda")
+        else:
+            aux = torch.empty(max(int(size.value), 1), dtype=torch.uint8, device="cuda")
+    if stream is not None and stream != torch.cuda.current_stream():
+        aux.record_stream(stream)
     cap = ctypes.c_size_t(aux.numel() * aux.element_size())
     _check(fn(_ptr(aux), ctypes.byref(cap), _stream(stream), *args), where)
     return aux

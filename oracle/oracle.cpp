@@ -453,13 +453,16 @@ void oracle_stable_sort(int kt, void* keys, uint32_t* vals, int64_t n, int great
   });
 }
 
-// One std::stable_sort per segment [segs[i], segs[i+1]) (last ends at n).
+// One std::stable_sort per segment [segs[i], segs[i+1]) (last ends at n),
+// both ends clamped to [0, n]: a start at or beyond n gives an empty segment.
 void oracle_segmented_stable_sort(int kt, void* keys, uint32_t* vals, int64_t n,
                                   const int32_t* segs, int64_t num_segments, int greater) {
   dispatch_key(kt, [&](auto* tag) {
     typedef std::remove_pointer_t<decltype(tag)> K;
     for (int64_t i = 0; i < num_segments; ++i) {
-      int64_t b = segs[i], e = i + 1 < num_segments ? segs[i + 1] : n;
+      int64_t b = std::min<int64_t>(std::max<int64_t>(segs[i], 0), n);
+      int64_t e = i + 1 < num_segments ? segs[i + 1] : n;
+      e = std::min<int64_t>(std::max<int64_t>(e, b), n);
       if (e > b) stable_sort_kv((K*)keys + b, vals ? vals + b : nullptr, e - b, greater);
     }
   });

@@ -19,3 +19,17 @@ def test_header_surface_program():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     assert "ALL PASS" in r.stdout
+
+
+def test_capture_before_first_call_program():
+    """A template caller that begins a global-mode graph capture before any
+    library call (tests/cpp/capture_first.hip, a fresh process): nothing the
+    library does on first use may allocate or synchronise inside the
+    capture, and three replays match the std:: algorithms."""
+    exe = os.path.join(os.path.dirname(EXE), "capture_first")
+    if not os.path.exists(exe):
+        raise RuntimeError("%s is not built (run __graft_entry__.build())" % exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "ALL PASS" in r.stdout

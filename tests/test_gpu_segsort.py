@@ -139,3 +139,34 @@ def test_segsort_graph_replay():
         stream.synchronize()
         np.testing.assert_array_equal(to_host(dk), expect[r][0])
         np.testing.assert_array_equal(to_host(dv), expect[r][1])
+
+
+@pytest.mark.parametrize("kv", [True, False])
+def test_segsort_starts_past_the_end(kv):
+    """Starts at or beyond count are empty segments (as in the segmented
+    scan): the sort equals the oracle's over the same starts, a segment that
+    reaches count goes through the large-segment radix path, and guard words
+    around the arrays stay untouched."""
+    n = 50000
+    banner("segsort starts past the end", n, kv)
+    nv = 512 * 17
+    starts = np.array([0, 7, 7, 7, nv, nv, n // 2, n, n, n + 9], dtype=np.int32)
+    keys = make_keys("u32", n, 31)
+    vals = np.arange(n, dtype=np.uint32)
+    guard = 64
+    gk = torch.full((n + 2 * guard,), 0x5A5A5A5A, dtype=torch.int32, device="cuda")
+    gk[guard:guard + n] = to_dev(keys.view(np.int32))
+    gv = torch.full((n + 2 * guard,), 0x3C3C3C3C, dtype=torch.int32, device="cuda")
+    gv[guard:guard + n] = to_dev(vals.view(np.int32))
+    dk = gk[guard:guard + n].view(torch.uint32)
+    dv = gv[guard:guard + n].view(torch.uint32) if kv else None
+    mgpu_hip.segmented_sort(dk, to_dev(starts), dv)
+    hk, hv = to_host(gk), to_host(gv)
+    assert np.all(hk[:guard] == 0x5A5A5A5A) and np.all(hk[guard + n:] == 0x5A5A5A5A)
+    assert np.all(hv[:guard] == 0x3C3C3C3C) and np.all(hv[guard + n:] == 0x3C3C3C3C)
+    if kv:
+        ek, ev = oracle.segmented_stable_sort(keys, starts, vals, "u32")
+        np.testing.assert_array_equal(hv[guard:guard + n].view(np.uint32), ev)
+    else:
+        ek = oracle.segmented_stable_sort(keys, starts, None, "u32")
+    np.testing.assert_array_equal(hk[guard:guard + n].view(np.uint32), ek)

@@ -458,6 +458,25 @@ def test_segmented_stable_sort_vs_numpy_lexsort(kt, greater):
     np.testing.assert_array_equal(k, keys[order])
 
 
+def test_segmented_stable_sort_starts_past_the_end():
+    """Starts at or beyond n give empty segments: the result equals the sort
+    over the starts below n (numpy lexsort by segment id, then key), and a
+    start far past n reads and writes nothing outside the arrays."""
+    n = 5000
+    keys = oracle.generate(n, "u32", 21)[np.arange(n) % 37]
+    vals = np.arange(n, dtype=np.uint32)
+    starts = np.array([0, 7, 7, 1000, n // 2, n, n, n + 9, 2 ** 31 - 1], dtype=np.int32)
+    k, v = oracle.segmented_stable_sort(keys, starts, vals, "u32")
+    inside = starts[starts < n]
+    seg_id = np.searchsorted(inside, np.arange(n), side="right")
+    order = np.lexsort((keys, seg_id))
+    np.testing.assert_array_equal(v, vals[order])
+    np.testing.assert_array_equal(k, keys[order])
+    k2, v2 = oracle.segmented_stable_sort(keys, inside, vals, "u32")
+    np.testing.assert_array_equal(k, k2)
+    np.testing.assert_array_equal(v, v2)
+
+
 def test_oracle_rejects_values_of_the_wrong_length():
     """The oracle passes raw buffers to C: a value array shorter than its
     keys must be an error, not a read past its end."""
