@@ -67,7 +67,8 @@ int mgpu_last_hip_error(void);
 /* Library version: major*10000 + minor*100 + patch. */
 int mgpu_version(void);
 /* Creates the library's per-device state on the current device (the
- * look-back fallback counter: 256 bytes, allocated and zeroed once).
+ * look-back fallback counter: 256 bytes, allocated and zeroed once) and
+ * looks up the device address of mgpu_scan_i32_noalloc's scratch array.
  * Optional: any entry point creates what it needs on first use, except
  * while its stream is being captured into a graph, when it runs without the
  * counter (only the fallback count is lost).  Stands in for the reference's
@@ -165,6 +166,20 @@ MGPU_DECL_SCAN(f32, float)
  * tile's 64-bit sum and flag are published as two atomic words, each with
  * the flag beside one 32-bit half (vk/scan.hxx:10-115 counterpart).        */
 MGPU_DECL_SCAN(u64, uint64_t)
+
+/* mgpu_scan_i32 without aux and without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and bit-identical results as mgpu_scan_i32; the
+ * look-back status words live in a fixed 1 MiB device array that every GPU
+ * the library runs on holds until the process ends.  count up to
+ * 2,684,354,560 (131072 tiles of 20480 elements); beyond that
+ * MGPU_UNSUPPORTED, before any launch.  Calls that overlap on one GPU share
+ * that array and race: serialise them (one stream, or events).  The array's
+ * address is looked up once per device by mgpu_init or by the first call.   */
+mgpu_status_t mgpu_scan_i32_noalloc(mgpu_stream_t stream, const int32_t* input,
+                                    int32_t* output, int64_t count,
+                                    mgpu_scan_type_t type, int32_t init);
 
 /* Segmented scan (no counterpart in the reference, whose scan has no
  * segmented form): one scan per segment, each starting from init.

@@ -7,6 +7,10 @@
 
 using namespace mgpu::hip;
 
+// Device address of the no-allocation scan's scratch on the current device
+// (capi_scan_noalloc.hip), looked up and kept per device.
+hipError_t mgpu_noalloc_scratch(uint64_t** out);
+
 extern "C" {
 
 const char* mgpu_status_string(mgpu_status_t status) {
@@ -24,7 +28,11 @@ int mgpu_last_hip_error(void) { return last_hip_error_slot(); }
 
 int mgpu_version(void) { return 100; }
 
-mgpu_status_t mgpu_init(void) { return init(); }
+mgpu_status_t mgpu_init(void) {
+  if (mgpu_status_t st = init()) return st;
+  uint64_t* scratch = nullptr;
+  return hip_status(mgpu_noalloc_scratch(&scratch));
+}
 
 void mgpu_timing_enable(int enable) {
   timing_registry_t& tr = timing_registry();
