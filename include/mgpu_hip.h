@@ -119,6 +119,28 @@ MGPU_DECL_MERGE(f32, float)
 MGPU_DECL_MERGE(i64, int64_t)
 MGPU_DECL_MERGE(f64, double)
 
+/* Merge-path partitions, vk::merge_path_partitions<bounds>(cmd, a, a_count,
+ * b, b_count, mp_data, spacing, comp) (inc/mgpu/vk/partition.hxx:9-20) with
+ * the stream in place of the command buffer and the bounds as an argument:
+ * mp[i] = the number of A elements among the first
+ * min(i * spacing, a_count + b_count) outputs of the merge of sorted a and b
+ * (merge_path, inc/mgpu/common/cta_merge.hxx:10-31), for
+ * i <= (a_count + b_count + spacing - 1) / spacing, i.e. the reference's
+ * num_merge_partitions (cta_merge.hxx:6-8) entries of int64.  LOWER: ties
+ * take A first (the stable merge); UPPER: ties take B first.  No aux; the
+ * caller sizes mp.  spacing > 0.                                          */
+typedef enum mgpu_bounds_t { MGPU_BOUNDS_LOWER = 0, MGPU_BOUNDS_UPPER = 1 } mgpu_bounds_t;
+#define MGPU_DECL_PARTITIONS(K, KT)                                            \
+  mgpu_status_t mgpu_merge_path_partitions_##K(mgpu_stream_t stream,         \
+      const KT* a, int64_t a_count, const KT* b, int64_t b_count, int64_t* mp, \
+      int64_t spacing, mgpu_comp_t comp, mgpu_bounds_t bounds);
+MGPU_DECL_PARTITIONS(u32, uint32_t)
+MGPU_DECL_PARTITIONS(i32, int32_t)
+MGPU_DECL_PARTITIONS(u64, uint64_t)
+MGPU_DECL_PARTITIONS(f32, float)
+MGPU_DECL_PARTITIONS(i64, int64_t)
+MGPU_DECL_PARTITIONS(f64, double)
+
 /* ------------------------------------------------------------------------ */
 /* LSD radix sort, 8-bit digits, ascending in radix-bit order (unsigned:
  * value order; signed: MSB flipped; float: -0.0 before +0.0, NaNs last),

@@ -43,3 +43,34 @@ MGPU_DEF_MERGE(u64, uint64_t)
 MGPU_DEF_MERGE(f32, float)
 MGPU_DEF_MERGE(i64, int64_t)
 MGPU_DEF_MERGE(f64, double)
+
+// Merge-path partitions (inc/mgpu/vk/partition.hxx:9-20 counterpart).
+template <bounds_t bounds, typename KT>
+static mgpu_status_t partitions_by_comp(hipStream_t s, const KT* a, int64_t a_count, const KT* b,
+                                        int64_t b_count, int64_t* mp, int64_t spacing,
+                                        mgpu_comp_t comp) {
+  if (comp == MGPU_LESS)
+    return merge_path_partitions<bounds>(s, a, a_count, b, b_count, mp, spacing, less_t<KT>());
+  if (comp == MGPU_GREATER)
+    return merge_path_partitions<bounds>(s, a, a_count, b, b_count, mp, spacing, greater_t<KT>());
+  return MGPU_INVALID_ARG;
+}
+
+#define MGPU_DEF_PARTITIONS(K, KT)                                                          \
+  extern "C" mgpu_status_t mgpu_merge_path_partitions_##K(                                  \
+      mgpu_stream_t stream, const KT* a, int64_t a_count, const KT* b, int64_t b_count,     \
+      int64_t* mp, int64_t spacing, mgpu_comp_t comp, mgpu_bounds_t bounds) {               \
+    hipStream_t s = (hipStream_t)stream;                                                    \
+    if (bounds == MGPU_BOUNDS_LOWER)                                                        \
+      return partitions_by_comp<bounds_lower>(s, a, a_count, b, b_count, mp, spacing, comp); \
+    if (bounds == MGPU_BOUNDS_UPPER)                                                        \
+      return partitions_by_comp<bounds_upper>(s, a, a_count, b, b_count, mp, spacing, comp); \
+    return MGPU_INVALID_ARG;                                                                \
+  }
+
+MGPU_DEF_PARTITIONS(u32, uint32_t)
+MGPU_DEF_PARTITIONS(i32, int32_t)
+MGPU_DEF_PARTITIONS(u64, uint64_t)
+MGPU_DEF_PARTITIONS(f32, float)
+MGPU_DEF_PARTITIONS(i64, int64_t)
+MGPU_DEF_PARTITIONS(f64, double)

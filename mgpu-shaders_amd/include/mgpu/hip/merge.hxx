@@ -18,35 +18,42 @@
 namespace mgpu {
 namespace hip {
 
-// Per-thread merge path (binary search), used in LDS.
-template <typename key_t, typename comp_t>
-__device__ inline int merge_path_lds(const key_t* a, int a_count, const key_t* b,
-                                     int b_count, int diag, comp_t comp) {
-  int begin = max_(0, diag - b_count);
-  int end = min_(diag, a_count);
+// Which input wins ties on a merge-path diagonal (common/cta_merge.hxx:18-31):
+// bounds_lower puts A's equal keys first (the stable merge of A then B);
+// bounds_upper puts B's first.
+enum bounds_t { bounds_lower, bounds_upper };
+
+// Number of A elements among the first `diag` outputs of the merge of sorted
+// a and b whose ties follow `bounds`: a binary search over the diagonal.
+template <bounds_t bounds = bounds_lower, typename index_t, typename key_t, typename comp_t>
+__host__ __device__ inline index_t merge_path_search(const key_t* a, index_t a_count,
+                                                     const key_t* b, index_t b_count,
+                                                     index_t diag, comp_t comp) {
+  index_t begin = diag > b_count ? diag - b_count : 0;
+  index_t end = diag < a_count ? diag : a_count;
   while (begin < end) {
-    int mid = (begin + end) >> 1;
-    bool pred = !comp(b[diag - 1 - mid], a[mid]);
+    const index_t mid = (begin + end) >> 1;
+    const bool pred = bounds == bounds_upper ? comp(a[mid], b[diag - 1 - mid])
+                                             : !comp(b[diag - 1 - mid], a[mid]);
     if (pred) begin = mid + 1;
     else end = mid;
   }
   return begin;
 }
 
+// Per-thread merge path (binary search), used in LDS.
+template <typename key_t, typename comp_t>
+__device__ inline int merge_path_lds(const key_t* a, int a_count, const key_t* b,
+                                     int b_count, int diag, comp_t comp) {
+  return merge_path_search<bounds_lower, int>(a, a_count, b, b_count, diag, comp);
+}
+
 // Merge path in global memory, one binary search per thread (the upper levels
 // of every search touch the same few cache lines, which stay in L2).
-template <typename key_t, typename comp_t>
+template <bounds_t bounds = bounds_lower, typename key_t, typename comp_t>
 __device__ inline int64_t merge_path_global(const key_t* a, int64_t a_count, const key_t* b,
                                             int64_t b_count, int64_t diag, comp_t comp) {
-  int64_t begin = max_<int64_t>(0, diag - b_count);
-  int64_t end = min_<int64_t>(diag, a_count);
-  while (begin < end) {
-    int64_t mid = (begin + end) >> 1;
-    bool pred = !comp(b[diag - 1 - mid], a[mid]);
-    if (pred) begin = mid + 1;
-    else end = mid;
-  }
-  return begin;
+  return merge_path_search<bounds, int64_t>(a, a_count, b, b_count, diag, comp);
 }
 
 // Serial merge of vt outputs starting at (a_begin, b_begin) of LDS keys, with
@@ -74,15 +81,47 @@ __device__ inline void serial_merge(const key_t* keys_shared, int a_begin, int a
 }
 
 // Partition kernel: mp[i] = merge_path(a, b, min(i * spacing, a_count + b_count))
-// (vk/partition.hxx:9-20).
-template <int NT, typename key_t, typename comp_t>
+// (vk/partition.hxx:9-20); mp_t is int64_t here, int for the reference's
+// int partition arrays.
+template <int NT, bounds_t bounds = bounds_lower, typename key_t, typename mp_t, typename comp_t>
 __global__ __launch_bounds__(NT) void kernel_merge_partition(
     const key_t* a, int64_t a_count, const key_t* b, int64_t b_count, int64_t spacing,
-    int64_t num_partitions, int64_t* mp, comp_t comp) {
+    int64_t num_partitions, mp_t* mp, comp_t comp) {
   int64_t index = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (index >= num_partitions) return;
   int64_t diag = min_<int64_t>(spacing * index, a_count + b_count);
-  mp[index] = merge_path_global(a, a_count, b, b_count, diag, comp);
+  mp[index] = (mp_t)merge_path_global<bounds>(a, a_count, b, b_count, diag, comp);
+}
+
+// Partitions of a merge into `spacing`-output tiles: the reference's
+// num_merge_partitions (common/cta_merge.hxx:6-8).
+constexpr int64_t num_merge_partitions(int64_t count, int64_t spacing) {
+  return (count + spacing - 1) / spacing + 1;
+}
+
+// vk::merge_path_partitions<bounds>(cmd, a, a_count, b, b_count, mp_data,
+// spacing, comp) (vk/partition.hxx:9-20) with the stream in place of the
+// command buffer: mp_data[i] = the merge path of diagonal
+// min(i * spacing, a_count + b_count) for i < num_merge_partitions(a_count +
+// b_count, spacing).  No aux.  mp_data is int (as in the reference: totals
+// below 2^31) or int64_t.
+template <bounds_t bounds, typename key_t, typename mp_t, typename comp_t>
+mgpu_status_t merge_path_partitions(hipStream_t stream, const key_t* a, int64_t a_count,
+                                    const key_t* b, int64_t b_count, mp_t* mp_data,
+                                    int64_t spacing, comp_t comp) {
+  static_assert(std::is_same_v<mp_t, int> || std::is_same_v<mp_t, int64_t>,
+                "partitions are int or int64_t");
+  if (a_count < 0 || b_count < 0 || spacing <= 0) return MGPU_INVALID_ARG;
+  if (std::is_same_v<mp_t, int> && a_count + b_count >= ((int64_t)1 << 31))
+    return MGPU_UNSUPPORTED;
+  if (!mp_data || (a_count && !a) || (b_count && !b)) return MGPU_INVALID_ARG;
+  const int64_t num_partitions = num_merge_partitions(a_count + b_count, spacing);
+  constexpr int PNT = 256;
+  return hip_status(launch_kernel("mgpu_merge_partition",
+                                  kernel_merge_partition<PNT, bounds, key_t, mp_t, comp_t>,
+                                  dim3((unsigned)div_up(num_partitions, (int64_t)PNT)), dim3(PNT),
+                                  0, stream, a, a_count, b, b_count, spacing, num_partitions,
+                                  mp_data, comp));
 }
 
 // LDS footprint of a tile merge.
@@ -191,8 +230,11 @@ __global__ __launch_bounds__(NT) void kernel_merge(
 }
 
 // Stable merge of (a_keys, a_vals) and (b_keys, b_vals) into (c_keys, c_vals);
-// val_t = empty_t for key-only.  inc/mgpu/vk/merge.hxx:12-59.
-template <int NT = 256, int VT = 16, typename key_t, typename val_t, typename comp_t>
+// val_t = empty_t for key-only.  inc/mgpu/vk/merge.hxx:12-59.  The default
+// tile is the one measured fastest for 32-bit keys on MI355X (512 x 23); VT
+// is odd because the thread-order LDS transposes read word VT * tid + i,
+// which a power-of-two VT puts in few banks.
+template <int NT = 512, int VT = 23, typename key_t, typename val_t, typename comp_t>
 mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
                     const key_t* a_keys, const val_t* a_vals, int64_t a_count,
                     const key_t* b_keys, const val_t* b_vals, int64_t b_count,
@@ -217,7 +259,8 @@ mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
   void* p = aux_data;
   int64_t* mp = advance_pointer<int64_t>(p, num_partitions);
   constexpr int PNT = 256;
-  hipError_t e = launch_kernel("mgpu_merge_partition", kernel_merge_partition<PNT, key_t, comp_t>,
+  hipError_t e = launch_kernel("mgpu_merge_partition",
+                               kernel_merge_partition<PNT, bounds_lower, key_t, int64_t, comp_t>,
                                dim3((unsigned)div_up(num_partitions, (int64_t)PNT)),
                                dim3(PNT), 0, stream, a_keys, a_count, b_keys, b_count,
                                (int64_t)NV, num_partitions, mp, comp);
@@ -229,7 +272,7 @@ mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
 }
 
 // Key-only merge (inc/mgpu/vk/merge.hxx:49-59; an explicit comparator).
-template <int NT = 256, int VT = 16, typename key_t, typename comp_t>
+template <int NT = 512, int VT = 23, typename key_t, typename comp_t>
 mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
                     const key_t* a_keys, int64_t a_count, const key_t* b_keys,
                     int64_t b_count, key_t* c_keys, comp_t comp) {

@@ -75,6 +75,7 @@ def load():
             ("mgpu_merge_keys_%s", [vp, sz, vp, vp, i64, vp, i64, vp, c_int]),
             ("mgpu_merge_kv_%s_u32", [vp, sz, vp, vp, vp, i64, vp, vp, i64, vp, vp, c_int]),
             ("mgpu_radix_sort_keys_%s", [vp, sz, vp, vp, i64]),
+            ("mgpu_merge_path_partitions_%s", [vp, vp, i64, vp, i64, vp, i64, c_int, c_int]),
             ("mgpu_radix_sort_kv_%s_u32", [vp, sz, vp, vp, vp, i64]),
         ]:
             f = getattr(lib, name % k)
@@ -273,6 +274,42 @@ def merge(a_keys, a_vals, b_keys, b_vals, c_keys, c_vals, comp, key_type=None, a
     return _run_two_phase(fn, [_ptr(a_keys), _ptr(a_vals), a_keys.numel(), _ptr(b_keys),
                                _ptr(b_vals), b_keys.numel(), _ptr(c_keys), _ptr(c_vals),
                                _comp(comp)], "merge", aux, stream)
+
+
+def num_merge_partitions(count, spacing):
+    """Entries of a partition array (inc/mgpu/common/cta_merge.hxx:6-8)."""
+    return (count + spacing - 1) // spacing + 1
+
+
+def merge_path_partitions(a_keys, b_keys, spacing, comp="less", bounds="lower", mp=None,
+                          key_type=None, stream=None):
+    """vk::merge_path_partitions<bounds> (inc/mgpu/vk/partition.hxx:9-20):
+    mp[i] = A elements among the first min(i * spacing, |a| + |b|) outputs of
+    the merge (ties take A first with bounds="lower", B first with "upper").
+    Returns mp, an int64 device tensor of num_merge_partitions entries."""
+    _device_check(a_keys, b_keys)
+    if a_keys.dtype != b_keys.dtype:
+        raise ValueError("a and b keys must have one dtype")
+    if spacing <= 0:
+        raise ValueError("spacing must be positive")
+    if bounds not in ("lower", "upper"):
+        raise ValueError("bounds must be 'lower' or 'upper'")
+    k = _key_type(a_keys, key_type)
+    n = num_merge_partitions(a_keys.numel() + b_keys.numel(), spacing)
+    if mp is None:
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                mp = torch.empty(n, dtype=torch.int64, device="cuda")
+        else:
+            mp = torch.empty(n, dtype=torch.int64, device="cuda")
+    elif mp.dtype != torch.int64 or mp.numel() < n:
+        raise ValueError("mp must be int64 with at least %d entries" % n)
+    _device_check(mp)
+    fn = getattr(load(), "mgpu_merge_path_partitions_%s" % k)
+    _check(fn(_stream(stream), _ptr(a_keys), a_keys.numel(), _ptr(b_keys), b_keys.numel(),
+              _ptr(mp), spacing, _comp(comp), 0 if bounds == "lower" else 1),
+           "merge_path_partitions")
+    return mp
 
 
 # ---------------------------------------------------------------------------

@@ -33,3 +33,16 @@ def test_capture_before_first_call_program():
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     assert "ALL PASS" in r.stdout
+
+
+def test_context_port_program():
+    """A call site in the reference's host idiom (context_t, cmd_buffer_t,
+    memcache_t, int merge-path partitions) ported to mgpu/hip
+    (tests/cpp/context_port.hip)."""
+    exe = os.path.join(os.path.dirname(EXE), "context_port")
+    if not os.path.exists(exe):
+        raise RuntimeError("%s is not built (run __graft_entry__.build())" % exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "ALL PASS" in r.stdout

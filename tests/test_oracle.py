@@ -313,6 +313,25 @@ def test_merge_path_brute_force():
             assert oracle.merge_path(a, b, diag) == brute_merge_path(a, b, diag)
 
 
+def brute_merge_path_upper(a, b, diag):
+    """A-count among the first diag outputs of the merge whose ties take B
+    first: the stable merge of (b, a), counting the a elements."""
+    tags = np.concatenate([np.ones(len(b), np.int8), np.zeros(len(a), np.int8)])
+    keys = np.concatenate([b, a])
+    order = np.argsort(keys, kind="stable")
+    return int((tags[order][:diag] == 0).sum())
+
+
+def test_merge_path_upper_brute_force():
+    rng = np.random.default_rng(1)
+    for trial in range(300):
+        na, nb = rng.integers(0, 40, size=2)
+        a = np.sort(rng.integers(0, 6, na)).astype(np.uint32)
+        b = np.sort(rng.integers(0, 6, nb)).astype(np.uint32)
+        for diag in range(na + nb + 1):
+            assert oracle.merge_path(a, b, diag, upper=True) == brute_merge_path_upper(a, b, diag)
+
+
 def test_merge_path_planted_upper_bound_is_caught():
     a = np.array([1, 2, 2, 3], np.uint32)
     b = np.array([2, 2, 4], np.uint32)
