@@ -277,9 +277,11 @@ void mgpu_graph_destroy(mgpu_graph_t graph);
 /* Serving engine (no counterpart in the reference): continuous batching of
  * sort / merge / scan requests over `slots` streams.  A request is admitted
  * as soon as a slot is free and its device bytes fit the HBM budget (first
- * fit from the head of the queue); its input is generated on the device from
- * (kind per key type, seed), and its result (and, with keep_inputs, its
- * input) stays device-resident until mgpu_server_release.  Requests:
+ * fit from the head of the queue).  It either brings its own device buffers
+ * (mgpu_server_submit_buffers) or, as a load generator (mgpu_server_submit),
+ * has its input generated on the device from (kind per key type, seed); a
+ * generated request's result (and, with keep_inputs, its input) stays
+ * device-resident until mgpu_server_release.  Generated requests:
  *   SORT   stable key-value LSD radix sort of n keys, values = index;
  *          keys uniform (f32: finite, U[-1, 1))
  *   MERGE  stable key-value merge of two ascending arrays of n keys each
@@ -315,6 +317,32 @@ double mgpu_server_now(mgpu_server_t server);
 /* Device bytes the server budgets for a request.                          */
 size_t mgpu_server_request_bytes(const mgpu_request_t* req);
 mgpu_status_t mgpu_server_submit(mgpu_server_t server, const mgpu_request_t* req, int64_t* id);
+/* A request on the caller's own device buffers (req->seed is unused): the
+ * server allocates only the call's aux (which alone counts against the
+ * budget), enqueues the work on a slot's stream when admitted, and reports
+ * the caller's pointers in the response.  The caller keeps every buffer
+ * alive and unmodified until the response is polled.
+ *   SORT   in_a (n keys) -> out, sorted stably (out may be in_a).  With
+ *          vals_out: values travel with the keys, from vals_a (may equal
+ *          vals_out) or, if vals_a is NULL, the original index.
+ *   MERGE  in_a (n keys) and in_b (n_b keys, n if n_b == 0), ascending ->
+ *          out (n + n_b keys); with vals_out, vals_a and vals_b are merged
+ *          alongside (ties take A first).
+ *   SCAN   in_a (n) -> out (may be in_a), scan_type exclusive or inclusive,
+ *          from 0; integers modulo 2^bits, f32 within the scan's 2e-6 bound. */
+typedef struct mgpu_request_buffers_t {
+  const void* in_a;
+  const void* in_b;
+  const uint32_t* vals_a;
+  const uint32_t* vals_b;
+  void* out;
+  uint32_t* vals_out;
+  int64_t n_b;
+  int32_t scan_type;  /* mgpu_scan_type_t */
+  int32_t reserved;
+} mgpu_request_buffers_t;
+mgpu_status_t mgpu_server_submit_buffers(mgpu_server_t server, const mgpu_request_t* req,
+                                         const mgpu_request_buffers_t* bufs, int64_t* id);
 /* Completed responses not yet reported (non-blocking); returns how many.  */
 int mgpu_server_poll(mgpu_server_t server, mgpu_response_t* out, int max);
 /* Blocks until every submitted request has completed (MGPU_OK), or until

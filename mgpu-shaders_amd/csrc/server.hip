@@ -34,6 +34,8 @@ typedef std::chrono::steady_clock clock_t_;
 struct rec_t {
   int64_t id = 0;
   mgpu_request_t req{};
+  bool caller = false;            // buffers owned by the caller (bufs)
+  mgpu_request_buffers_t bufs{};
   double t_submit = 0, t_start = 0, t_done = 0;
   int slot = -1;
   mgpu_status_t status = MGPU_OK;
@@ -121,6 +123,124 @@ size_t request_bytes(const mgpu_request_t& r) {
 hipError_t gen(hipStream_t s, void* out, int64_t n, int eb, int kind, uint64_t seed) {
   return generate(out, n, eb, kind, seed, 0, s) == MGPU_OK ? hipSuccess : hipErrorLaunchFailure;
 }
+
+// Aux bytes of a request on caller buffers (the only device memory the
+// server allocates for it).
+size_t caller_aux_bytes(const mgpu_request_t& r, const mgpu_request_buffers_t& b) {
+  const int64_t n = r.n;
+  const int64_t nb = b.n_b > 0 ? b.n_b : n;
+  size_t aux = 0;
+  switch (r.op) {
+    case MGPU_REQ_SORT:
+      if (r.key_type == MGPU_KEY_U64) {
+        if (b.vals_out) radix_sort_tuned(nullptr, aux, nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr, n);
+        else radix_sort_tuned(nullptr, aux, nullptr, (uint64_t*)nullptr, (empty_t*)nullptr, n);
+      } else {
+        if (b.vals_out) radix_sort_tuned(nullptr, aux, nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, n);
+        else radix_sort_tuned(nullptr, aux, nullptr, (uint32_t*)nullptr, (empty_t*)nullptr, n);
+      }
+      break;
+    case MGPU_REQ_MERGE:
+      // Partitions only: the same for keys and key-value merges.
+      if (r.key_type == MGPU_KEY_U64)
+        merge<merge_tuning<uint64_t>::nt, merge_tuning<uint64_t>::vt>(
+            nullptr, aux, nullptr, (const uint64_t*)nullptr, n, (const uint64_t*)nullptr, nb,
+            (uint64_t*)nullptr, less_t<uint64_t>());
+      else
+        merge<merge_tuning<uint32_t>::nt, merge_tuning<uint32_t>::vt>(
+            nullptr, aux, nullptr, (const uint32_t*)nullptr, n, (const uint32_t*)nullptr, nb,
+            (uint32_t*)nullptr, less_t<uint32_t>());
+      break;
+    case MGPU_REQ_SCAN:
+      if (r.key_type == MGPU_KEY_U64)
+        scan64<uint64_t>(nullptr, aux, nullptr, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
+      else
+        scan<scan_tuning::nt, scan_tuning::vt>(nullptr, aux, nullptr, (const uint32_t*)nullptr,
+                                               (uint32_t*)nullptr, n, MGPU_SCAN_EXCLUSIVE, 0u);
+      break;
+  }
+  return aux_round(aux ? aux : 1);
+}
+
+template <typename K>
+mgpu_status_t caller_sort(rec_t* r, hipStream_t s, size_t cap) {
+  const mgpu_request_buffers_t& b = r->bufs;
+  const int64_t n = r->req.n;
+  hipError_t e = hipSuccess;
+  if (b.out != b.in_a) e = hipMemcpyAsync(b.out, b.in_a, n * sizeof(K), hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return hip_status(e);
+  if (!b.vals_out) return radix_sort_tuned(r->aux, cap, s, (K*)b.out, (empty_t*)nullptr, n);
+  if (b.vals_a) {
+    if (b.vals_a != b.vals_out)
+      e = hipMemcpyAsync(b.vals_out, b.vals_a, n * 4, hipMemcpyDeviceToDevice, s);
+  } else {
+    e = gen(s, b.vals_out, n, 4, gen_iota, 0);
+  }
+  if (e != hipSuccess) return hip_status(e);
+  return radix_sort_tuned(r->aux, cap, s, (K*)b.out, b.vals_out, n);
+}
+
+template <typename K>
+mgpu_status_t caller_merge(rec_t* r, hipStream_t s, size_t cap) {
+  const mgpu_request_buffers_t& b = r->bufs;
+  const int64_t n = r->req.n, nb = b.n_b > 0 ? b.n_b : n;
+  constexpr int NT = merge_tuning<K>::nt, VT = merge_tuning<K>::vt;
+  if (!b.vals_out)
+    return merge<NT, VT>(r->aux, cap, s, (const K*)b.in_a, n, (const K*)b.in_b, nb, (K*)b.out,
+                         less_t<K>());
+  return merge<NT, VT>(r->aux, cap, s, (const K*)b.in_a, b.vals_a, n, (const K*)b.in_b, b.vals_b,
+                       nb, (K*)b.out, b.vals_out, less_t<K>());
+}
+
+// Enqueue a request on caller-owned buffers: sort copies the input into the
+// output (unless they are the same) and sorts there, values from vals_a or
+// the index; merge and scan write their output directly.
+mgpu_status_t enqueue_caller(rec_t* r, hipStream_t s) {
+  const mgpu_request_t& q = r->req;
+  const mgpu_request_buffers_t& b = r->bufs;
+  const int64_t n = q.n;
+  const size_t cap = caller_aux_bytes(q, b);
+  hipError_t e = hipMallocAsync(&r->aux, cap, s);
+  if (e != hipSuccess) return hip_status(e);
+  switch (q.op) {
+    case MGPU_REQ_SORT:
+      switch (q.key_type) {
+        case MGPU_KEY_U64: return caller_sort<uint64_t>(r, s, cap);
+        case MGPU_KEY_I32: return caller_sort<int32_t>(r, s, cap);
+        case MGPU_KEY_F32: return caller_sort<float>(r, s, cap);
+        default: return caller_sort<uint32_t>(r, s, cap);
+      }
+    case MGPU_REQ_MERGE:
+      switch (q.key_type) {
+        case MGPU_KEY_U64: return caller_merge<uint64_t>(r, s, cap);
+        case MGPU_KEY_I32: return caller_merge<int32_t>(r, s, cap);
+        case MGPU_KEY_F32: return caller_merge<float>(r, s, cap);
+        default: return caller_merge<uint32_t>(r, s, cap);
+      }
+    case MGPU_REQ_SCAN: {
+      size_t c = cap;
+      switch (q.key_type) {
+        case MGPU_KEY_U64:
+          return scan64<uint64_t>(r->aux, c, s, (const uint64_t*)b.in_a, (uint64_t*)b.out, n,
+                                  (mgpu_scan_type_t)b.scan_type, 0ull);
+        case MGPU_KEY_F32:
+          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, c, s, (const float*)b.in_a,
+                                                        (float*)b.out, n,
+                                                        (mgpu_scan_type_t)b.scan_type, 0.0f);
+        case MGPU_KEY_I32:
+          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, c, s, (const int32_t*)b.in_a,
+                                                        (int32_t*)b.out, n,
+                                                        (mgpu_scan_type_t)b.scan_type, 0);
+        default:
+          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, c, s, (const uint32_t*)b.in_a,
+                                                        (uint32_t*)b.out, n,
+                                                        (mgpu_scan_type_t)b.scan_type, 0u);
+      }
+    }
+  }
+  return MGPU_INVALID_ARG;
+}
+
 
 hipError_t alloc(rec_t* r, void** p, size_t bytes, hipStream_t s) {
   if (bytes == 0) bytes = 256;
@@ -229,7 +349,7 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
 
 void free_buffers(mgpu_server_st* sv, rec_t* r) {
   for (void** p : {&r->in_a, &r->in_b, &r->out, &r->vals, &r->vals_a, &r->aux})
-    if (*p) {
+    if (*p && (!r->caller || p == &r->aux)) {
       (void)hipFreeAsync(*p, sv->free_stream);
       *p = nullptr;
     }
@@ -252,7 +372,7 @@ void worker_loop(mgpu_server_st* sv) {
       if (q == hipErrorNotReady) continue;
       r->t_done = sv->now();
       if (q != hipSuccess && r->status == MGPU_OK) r->status = hip_status(q);
-      if (!sv->keep_inputs) {
+      if (!sv->keep_inputs && !r->caller) {
         for (void** p : {&r->in_a, &r->in_b, &r->vals_a})
           if (*p) { (void)hipFreeAsync(*p, sv->free_stream); *p = nullptr; }
       }
@@ -277,7 +397,7 @@ void worker_loop(mgpu_server_st* sv) {
       r->slot = (int)i;
       r->t_start = sv->now();
       hipStream_t s = sv->streams[i];
-      r->status = enqueue(r, s);
+      r->status = r->caller ? enqueue_caller(r, s) : enqueue(r, s);
       (void)hipEventRecord(r->done, s);
       sv->slot_rec[i] = r;
       ++sv->in_flight;
@@ -338,14 +458,30 @@ size_t mgpu_server_request_bytes(const mgpu_request_t* req) {
   return req ? request_bytes(*req) : 0;
 }
 
-mgpu_status_t mgpu_server_submit(mgpu_server_t sv, const mgpu_request_t* req, int64_t* id) {
+static mgpu_status_t submit(mgpu_server_t sv, const mgpu_request_t* req,
+                            const mgpu_request_buffers_t* bufs, int64_t* id) {
   if (!sv || !req || req->n < 1) return MGPU_INVALID_ARG;
   if (req->op < MGPU_REQ_SORT || req->op > MGPU_REQ_SCAN) return MGPU_INVALID_ARG;
   if (req->key_type < MGPU_KEY_U32 || req->key_type > MGPU_KEY_F32) return MGPU_INVALID_ARG;
   if (req->n >= ((int64_t)1 << 30)) return MGPU_UNSUPPORTED;
+  if (bufs) {
+    if (!bufs->in_a || !bufs->out) return MGPU_INVALID_ARG;
+    if (bufs->n_b < 0 || bufs->n_b >= ((int64_t)1 << 30)) return MGPU_INVALID_ARG;
+    if (req->op == MGPU_REQ_MERGE) {
+      if (!bufs->in_b) return MGPU_INVALID_ARG;
+      if (bufs->vals_out && (!bufs->vals_a || !bufs->vals_b)) return MGPU_INVALID_ARG;
+    }
+    if (req->op == MGPU_REQ_SCAN && bufs->scan_type != MGPU_SCAN_EXCLUSIVE &&
+        bufs->scan_type != MGPU_SCAN_INCLUSIVE)
+      return MGPU_INVALID_ARG;
+  }
   rec_t* r = new rec_t();
   r->req = *req;
-  r->bytes = request_bytes(*req);
+  if (bufs) {
+    r->caller = true;
+    r->bufs = *bufs;
+  }
+  r->bytes = bufs ? caller_aux_bytes(*req, *bufs) : request_bytes(*req);
   if (r->bytes > sv->budget) {
     delete r;
     return MGPU_UNSUPPORTED;
@@ -364,6 +500,16 @@ mgpu_status_t mgpu_server_submit(mgpu_server_t sv, const mgpu_request_t* req, in
   return MGPU_OK;
 }
 
+mgpu_status_t mgpu_server_submit(mgpu_server_t sv, const mgpu_request_t* req, int64_t* id) {
+  return submit(sv, req, nullptr, id);
+}
+
+mgpu_status_t mgpu_server_submit_buffers(mgpu_server_t sv, const mgpu_request_t* req,
+                                         const mgpu_request_buffers_t* bufs, int64_t* id) {
+  if (!bufs) return MGPU_INVALID_ARG;
+  return submit(sv, req, bufs, id);
+}
+
 int mgpu_server_poll(mgpu_server_t sv, mgpu_response_t* out, int max) {
   if (!sv) return 0;
   std::lock_guard<std::mutex> lock(sv->m);
@@ -377,10 +523,17 @@ int mgpu_server_poll(mgpu_server_t sv, mgpu_response_t* out, int max) {
     o.op = r->req.op;
     o.key_type = r->req.key_type;
     o.n = r->req.n;
-    o.keys = r->out;
-    o.vals = r->vals;
-    o.input_a = r->in_a;
-    o.input_b = r->in_b;
+    if (r->caller) {
+      o.keys = r->bufs.out;
+      o.vals = r->bufs.vals_out;
+      o.input_a = (void*)r->bufs.in_a;
+      o.input_b = (void*)r->bufs.in_b;
+    } else {
+      o.keys = r->out;
+      o.vals = r->vals;
+      o.input_a = r->in_a;
+      o.input_b = r->in_b;
+    }
     o.t_submit_ms = r->t_submit;
     o.t_start_ms = r->t_start;
     o.t_done_ms = r->t_done;

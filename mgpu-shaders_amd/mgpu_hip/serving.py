@@ -29,6 +29,14 @@ class Request(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class RequestBuffers(ctypes.Structure):
+    _fields_ = [("in_a", ctypes.c_void_p), ("in_b", ctypes.c_void_p),
+                ("vals_a", ctypes.c_void_p), ("vals_b", ctypes.c_void_p),
+                ("out", ctypes.c_void_p), ("vals_out", ctypes.c_void_p),
+                ("n_b", ctypes.c_int64), ("scan_type", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class Response(ctypes.Structure):
     _fields_ = [("id", ctypes.c_int64), ("status", ctypes.c_int32), ("op", ctypes.c_int32),
                 ("key_type", ctypes.c_int32), ("slot", ctypes.c_int32), ("n", ctypes.c_int64),
@@ -51,6 +59,10 @@ def _bind(lib):
     lib.mgpu_server_request_bytes.argtypes = [ctypes.POINTER(Request)]
     lib.mgpu_server_submit.restype = ctypes.c_int
     lib.mgpu_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_int64)]
+    lib.mgpu_server_submit_buffers.restype = ctypes.c_int
+    lib.mgpu_server_submit_buffers.argtypes = [vp, ctypes.POINTER(Request),
+                                               ctypes.POINTER(RequestBuffers),
+                                               ctypes.POINTER(ctypes.c_int64)]
     lib.mgpu_server_poll.restype = ctypes.c_int
     lib.mgpu_server_poll.argtypes = [vp, ctypes.POINTER(Response), ctypes.c_int]
     lib.mgpu_server_drain.restype = ctypes.c_int
@@ -93,6 +105,34 @@ class Server:
         rid = ctypes.c_int64(-1)
         _check(self.lib.mgpu_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)),
                "server_submit")
+        return rid.value
+
+    def submit_buffers(self, op, key_type, in_a, out, in_b=None, vals_a=None, vals_b=None,
+                       vals_out=None, inclusive=False):
+        """A request on caller-owned device tensors (kept alive and unmodified
+        by the caller until the response is polled); see mgpu_server_submit_buffers."""
+        for t in (in_a, out, in_b, vals_a, vals_b, vals_out):
+            if t is not None and (not t.is_cuda or not t.is_contiguous()):
+                raise ValueError("buffers must be contiguous device tensors")
+        n = in_a.numel()
+        nb = in_b.numel() if in_b is not None else 0
+        m = n + nb if op == "merge" else n
+        if out.numel() < m or out.element_size() != in_a.element_size():
+            raise ValueError("out must hold %d keys of the input's size" % m)
+        if vals_out is not None and (vals_out.numel() < m or vals_out.element_size() != 4):
+            raise ValueError("vals_out must hold %d 32-bit values" % m)
+        for v, cnt in ((vals_a, n), (vals_b, nb)):
+            if v is not None and (v.numel() != cnt or v.element_size() != 4):
+                raise ValueError("values must be 32-bit, one per key")
+        if in_b is not None and in_b.element_size() != in_a.element_size():
+            raise ValueError("a and b keys must have one size")
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        r = Request(OPS[op], KEY[key_type], n, 0)
+        b = RequestBuffers(ptr(in_a), ptr(in_b), ptr(vals_a), ptr(vals_b), ptr(out), ptr(vals_out),
+                           nb, 1 if inclusive else 0, 0)
+        rid = ctypes.c_int64(-1)
+        _check(self.lib.mgpu_server_submit_buffers(self.h, ctypes.byref(r), ctypes.byref(b),
+                                                   ctypes.byref(rid)), "server_submit_buffers")
         return rid.value
 
     def poll(self, max_n=256):
