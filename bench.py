@@ -97,21 +97,23 @@ def load_traffic(workload):
         return None
 
 
-def cpu_baseline(log2n):
-    """std::stable_sort of (key, index) pairs on one core (the oracle)."""
+def cpu_baseline(log2n, reps=3):
+    """std::stable_sort of (key, index) pairs on one core (the oracle): the
+    median of `reps` runs over the same 2^log2n pairs (input generation and
+    the copy the oracle sorts are outside the timing)."""
     import numpy as np
     sys.path.insert(0, ROOT)
     import oracle
     n = 1 << log2n
     keys = oracle.generate(n, "u32", 0)
     vals = np.arange(n, dtype=np.uint32)
-    t0 = time.perf_counter()
-    oracle.stable_sort(keys, vals, "u32")
-    dt = time.perf_counter() - t0
+    ts = [oracle.stable_sort_timed(keys, vals, "u32") for _ in range(reps)]
+    dt = sorted(ts)[len(ts) // 2]
     return {"value": n / dt, "unit": "keys/s", "cores": 1, "kind": "port",
             "sample": "std::stable_sort of 2^%d (u32 key, u32 index) pairs, uniform keys "
-                      "(seed 0), %.2f s on one core, nproc=%d" % (log2n, dt, os.cpu_count() or 0),
-            "cpu_model": cpu_model()}
+                      "(seed 0), median of %d runs (%s s) on one core, nproc=%d"
+                      % (log2n, reps, ", ".join("%.2f" % t for t in ts), os.cpu_count() or 0),
+            "seconds": ts, "cpu_model": cpu_model()}
 
 
 def cpu_model():
@@ -370,16 +372,55 @@ def bench_segsort(torch, mgpu_hip, steps, warmup):
     ms_loop = (time.perf_counter() - t0) * 1e3
     same = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
                 torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
+
+    # The same loop of plain sorts recorded once into a graph and replayed:
+    # the GPU-side cost of one launch sequence per segment, without the host
+    # dispatch of each call.
+    lstream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    lsp = ctypes.c_void_p(lstream.cuda_stream)
+    with torch.cuda.stream(lstream):
+        lg = mgpu_hip.Graph(lstream)
+        with lg:
+            for kp, vp, l in work:
+                st = fn(auxp, ctypes.byref(cap), lsp, kp, vp, l, 0)
+                if st:
+                    raise RuntimeError("plain mergesort failed: %d" % st)
+        ms_loop_graph, _ = timed_steps(torch, 3, 1, restore, lg.launch)
+    torch.cuda.synchronize()
+    same_graph = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
+                      torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
+    del lg
+
+    # Algorithmic bytes: 16 B per key-value pair for every pass actually run:
+    # one read and one write of the segments the blocksort buckets hold, four
+    # digit passes over the segments longer than the largest bucket; segments
+    # of one key are not touched.
+    cap_max = max(tuning["cap"])
+    small = int(lens[(lens > 1) & (lens <= cap_max)].sum())
+    large = int(lens[lens > cap_max].sum())
+    alg_bytes = 16 * small + 16 * 4 * large
     shapes = mgpu_hip.segsort_shapes()
     out = {"segments": 65536, "keys": n, "log2_keys": float(np.log2(n)),
            "ms_batched_eager": ms_eager, "ms_batched_graph": ms_graph,
            "keys_per_s_graph": n / (ms_graph / 1e3),
+           "roofline": {"bound": "hbm", "algorithmic_bytes_per_call": alg_bytes,
+                        "keys_in_buckets": small, "keys_in_large_segments": large,
+                        "achieved": alg_bytes / (ms_graph / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s",
+                        "frac": alg_bytes / (ms_graph / 1e3) / 1e9 / HBM_PEAK_GBS},
            "ms_loop_of_plain_sorts": ms_loop, "plain_sort_calls": len(work),
-           "speedup_vs_loop": ms_loop / ms_graph, "outputs_identical": same,
+           "ms_loop_of_plain_sorts_graph": ms_loop_graph,
+           "speedup_vs_loop": ms_loop / ms_graph,
+           "speedup_vs_loop_graph": ms_loop_graph / ms_graph,
+           "outputs_identical": same and same_graph,
            "tuning": {"cap": tuning["cap"], "shape": [list(shapes[s]) for s in tuning["shape"]]},
            "kernels_ms": {k2: round(v2[1], 4) for k2, v2 in kt.items()}}
-    log("segsort", {k2: out[k2] for k2 in ("ms_batched_graph", "ms_loop_of_plain_sorts",
-                                           "speedup_vs_loop", "outputs_identical")})
+    log("segsort", {k2: out[k2] for k2 in ("ms_batched_eager", "ms_batched_graph",
+                                           "ms_loop_of_plain_sorts",
+                                           "ms_loop_of_plain_sorts_graph", "speedup_vs_loop",
+                                           "speedup_vs_loop_graph", "outputs_identical")},
+        "frac", out["roofline"]["frac"])
     return out
 
 
@@ -698,7 +739,8 @@ def main():
                     help="skip the key-only radix sizes of the reference chart")
     ap.add_argument("--no-vendor", action="store_true",
                     help="skip the rocPRIM comparison (vendor_rocprim.so)")
-    ap.add_argument("--cpu-log2", type=int, default=25)
+    ap.add_argument("--cpu-log2", type=int, default=27)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path "
@@ -786,8 +828,8 @@ def main():
         if not args.no_serving:
             result["serving"] = bench_serving(torch, mgpu_hip)
     if not args.no_cpu:
-        log("cpu baseline 2^%d" % args.cpu_log2)
-        result["cpu_baseline"] = cpu_baseline(args.cpu_log2)
+        log("cpu baseline 2^%d x %d" % (args.cpu_log2, args.cpu_reps))
+        result["cpu_baseline"] = cpu_baseline(args.cpu_log2, args.cpu_reps)
         if not args.no_secondary:
             result["cpu_baselines_other"] = cpu_baselines_other()
     print(json.dumps(result), flush=True)

@@ -111,7 +111,7 @@ struct mgpu_graph_st {
 
 mgpu_status_t mgpu_capture_begin(mgpu_stream_t stream) {
   if (!stream) return MGPU_INVALID_ARG;  // the legacy default stream cannot capture
-  (void)lookback_config();                // device-side state exists before capture
+  (void)init();  // per-device state (look-back counter, side streams) exists before capture
   timing_registry().enabled = false;
   return hip_status(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
 }

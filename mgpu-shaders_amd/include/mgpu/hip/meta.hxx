@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <atomic>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -247,12 +248,104 @@ inline lookback_config_t lookback_config_existing() {
   return lookback_config_t{s.spin_limit, c};
 }
 
+////////////////////////////////////////////////////////////////////////////////
+// Side streams for independent launches inside one call (the segmented
+// sort's size buckets and its large-segment radix touch disjoint segments):
+// fork_join_t makes up to max_side_streams streams wait for the work already
+// on the caller's stream, the call spreads its independent launches over
+// them, and join makes the caller's stream wait for all of them.  Under
+// graph capture the branches become parallel branches of the graph.  The
+// streams (non-blocking, default priority) and events are created once per
+// device, outside any capture; if they do not exist yet while the caller's
+// stream is being captured, every branch is the caller's stream (the same
+// work, in order).  One fork-join at a time uses a device's side streams.
+
+constexpr int max_side_streams = 8;
+
+struct side_streams_t {
+  struct device_t {
+    std::mutex use;
+    hipEvent_t fork = nullptr;
+    hipStream_t streams[max_side_streams] = {};
+    hipEvent_t joins[max_side_streams] = {};
+  };
+  std::mutex mutex;
+  std::vector<std::unique_ptr<device_t>> devices;
+};
+inline side_streams_t& side_streams() {
+  static side_streams_t s;
+  return s;
+}
+
+// The current device's side streams, created on first use unless `origin`
+// is being captured (then nullptr).
+inline side_streams_t::device_t* side_streams_for(hipStream_t origin) {
+  side_streams_t& ss = side_streams();
+  std::lock_guard<std::mutex> lock(ss.mutex);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  if ((int)ss.devices.size() <= dev) ss.devices.resize(dev + 1);
+  if (ss.devices[dev]) return ss.devices[dev].get();
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (origin && hipStreamIsCapturing(origin, &cs) == hipSuccess &&
+      cs != hipStreamCaptureStatusNone)
+    return nullptr;
+  std::unique_ptr<side_streams_t::device_t> d(new side_streams_t::device_t);
+  bool ok = hipEventCreateWithFlags(&d->fork, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; ok && i < max_side_streams; ++i)
+    ok = hipStreamCreateWithFlags(&d->streams[i], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&d->joins[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) return nullptr;  // (what was created stays until the process ends)
+  ss.devices[dev] = std::move(d);
+  return ss.devices[dev].get();
+}
+
+struct fork_join_t {
+  // Branches 0 .. n-1 start after the work on `origin` so far.
+  fork_join_t(hipStream_t origin, int n) : origin(origin) {
+    if (n <= 1) return;
+    dev = side_streams_for(origin);
+    if (!dev) return;
+    lock = std::unique_lock<std::mutex>(dev->use);
+    n = n < max_side_streams ? n : max_side_streams;
+    if (hipEventRecord(dev->fork, origin) != hipSuccess) return;
+    for (int i = 0; i < n; ++i)
+      if (hipStreamWaitEvent(dev->streams[i], dev->fork, 0) != hipSuccess) return;
+    active = n;
+  }
+  // The stream of branch i (the caller's stream when not forked).
+  hipStream_t branch(int i) const { return active ? dev->streams[i % active] : origin; }
+  // The caller's stream waits for every branch.
+  hipError_t join() {
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < active; ++i) {
+      hipError_t r = hipEventRecord(dev->joins[i], dev->streams[i]);
+      if (r == hipSuccess) r = hipStreamWaitEvent(origin, dev->joins[i], 0);
+      if (r != hipSuccess && e == hipSuccess) e = r;
+    }
+    active = 0;
+    if (lock.owns_lock()) lock.unlock();
+    return e;
+  }
+  ~fork_join_t() { (void)join(); }
+  fork_join_t(const fork_join_t&) = delete;
+  fork_join_t& operator=(const fork_join_t&) = delete;
+
+ private:
+  hipStream_t origin;
+  side_streams_t::device_t* dev = nullptr;
+  std::unique_lock<std::mutex> lock;
+  int active = 0;
+};
+
 // Creates the library's per-device state on the current device (the
-// look-back fallback counter).  Optional: every entry point creates it on
-// first use outside a graph capture; call it once before capturing if the
-// captured launches should count look-back fallbacks.
+// look-back fallback counter, the side streams).  Optional: every entry
+// point creates what it needs on first use outside a graph capture; call it
+// once before capturing if the captured launches should count look-back
+// fallbacks and a captured segmented sort should branch.
 inline mgpu_status_t init() {
-  return lookback_config().fallbacks ? MGPU_OK : MGPU_HIP_ERROR;
+  if (!lookback_config().fallbacks) return MGPU_HIP_ERROR;
+  return side_streams_for(nullptr) ? MGPU_OK : MGPU_HIP_ERROR;
 }
 
 ////////////////////////////////////////////////////////////////////////////////

@@ -359,10 +359,14 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
                     max_tiles);
   if (e != hipSuccess) return hip_status(e);
 
+  // The buckets and the large segments hold disjoint segments: each runs on
+  // a branch of its own, so one bucket's tail overlaps the others' work.
+  fork_join_t fork(stream, NB + 1);
+
   // Blocksort buckets.
   for (int b = 0; b < NB; ++b) {
     e = segsort_dispatch_shape(tuning.shape[b], keys, vals, count, segments, S, list,
-                               bounds + 2 * b, comp, stream);
+                               bounds + 2 * b, comp, fork.branch(b));
     if (e != hipSuccess) return hip_status(e);
   }
 
@@ -377,22 +381,23 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   // at the start of the scan aux; the count kernel zeroes them.
   uint64_t* scan_status = (uint64_t*)p;
   const int64_t scan_words = 2 * div_up(max_counts, (int64_t)(256 * 16));
+  const hipStream_t rs = fork.branch(NB);
   for (int pass = 0; pass < D; ++pass) {
     // Every count a task owns is rewritten each pass; entries past the last
     // task only follow the real ones in the scan, so they need no clearing.
     const int shift = 8 * pass;
     e = launch_kernel("mgpu_segsort_radix_count", kernel_segsort_radix_count<key_t, descending>,
-                      dim3(grid), dim3(segsort_radix_cnt), 0, stream, (const key_t*)k_in, count,
+                      dim3(grid), dim3(segsort_radix_cnt), 0, rs, (const key_t*)k_in, count,
                       segments, S, (const int2*)tasks, (const int32_t*)num_tasks,
                       (const uint64_t*)large_tiles, (const uint64_t*)tile_pos, shift, counts,
                       (uint32_t*)scan_status, scan_words);
     if (e != hipSuccess) return hip_status(e);
-    e = scan_launch_cleared<256, 16>(stream, (const uint32_t*)counts, counts, max_counts,
+    e = scan_launch_cleared<256, 16>(rs, (const uint32_t*)counts, counts, max_counts,
                                      MGPU_SCAN_EXCLUSIVE, 0u, identity_t(), scan_status);
     if (e != hipSuccess) return hip_status(e);
     e = launch_kernel("mgpu_segsort_radix_scatter",
                       kernel_segsort_radix_scatter<key_t, has_values, descending>, dim3(grid),
-                      dim3(segsort_radix_nt), 0, stream, (const key_t*)k_in, k_out,
+                      dim3(segsort_radix_nt), 0, rs, (const key_t*)k_in, k_out,
                       (const uint32_t*)v_in, v_out, count, segments, S, (const int2*)tasks,
                       (const int32_t*)num_tasks, (const uint64_t*)large_tiles,
                       (const uint64_t*)tile_pos, shift, (const uint32_t*)counts);
@@ -400,7 +405,7 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
     std::swap(k_in, k_out);
     std::swap(v_in, v_out);
   }
-  return MGPU_OK;
+  return hip_status(fork.join());
 }
 
 ////////////////////////////////////////////////////////////////////////////////

@@ -52,6 +52,8 @@ def lib():
         L.oracle_generate.argtypes = [vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
         L.oracle_glibc_rand.argtypes = [i64, vp]
         L.oracle_stable_sort.argtypes = [c_int, vp, vp, i64, c_int]
+        L.oracle_stable_sort_timed.argtypes = [c_int, vp, vp, i64, c_int]
+        L.oracle_stable_sort_timed.restype = ctypes.c_double
         L.oracle_stable_sort_radix.argtypes = [c_int, vp, vp, i64]
         L.oracle_segmented_stable_sort.argtypes = [c_int, vp, vp, i64, vp, i64, c_int]
         L.oracle_std_sort_u32.argtypes = [vp, i64]
@@ -151,6 +153,16 @@ def segmented_stable_sort(keys, segments, vals=None, kt="u32", greater=False):
     sg = np.ascontiguousarray(segments, dtype=np.int32)
     lib().oracle_segmented_stable_sort(KT[kt], _p(k), _p(v), len(k), _p(sg), len(sg), int(greater))
     return (k, v) if vals is not None else k
+
+
+def stable_sort_timed(keys, vals=None, kt="u32", greater=False):
+    """Seconds std::stable_sort takes on a copy of (keys, vals) (the CPU
+    baseline; copying and pair packing are not timed)."""
+    k = np.ascontiguousarray(keys, dtype=NP[kt]).copy()
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    if v is not None and len(v) != len(k):
+        raise ValueError("one value per key")
+    return float(lib().oracle_stable_sort_timed(KT[kt], _p(k), _p(v), len(k), int(greater)))
 
 
 def stable_sort_radix(keys, vals=None, kt="u32"):

@@ -32,6 +32,7 @@
 // reference's example inputs (oracle_glibc_rand).
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -106,19 +107,24 @@ struct comp_t {
 };
 
 // ----------------------------------------------------------- std:: oracles
+// Returns the seconds std::stable_sort itself took (steady_clock), packing
+// the (key, value) pairs and unpacking them excluded.
 template <typename K>
-void stable_sort_kv(K* keys, uint32_t* vals, int64_t n, int greater) {
+double stable_sort_kv(K* keys, uint32_t* vals, int64_t n, int greater) {
   std::vector<std::pair<K, uint32_t>> v((size_t)n);
   for (int64_t i = 0; i < n; ++i) v[i] = {keys[i], vals ? vals[i] : 0u};
   comp_t<K> c{greater};
+  const auto t0 = std::chrono::steady_clock::now();
   std::stable_sort(v.begin(), v.end(),
                    [&](const std::pair<K, uint32_t>& a, const std::pair<K, uint32_t>& b) {
                      return c(a.first, b.first);
                    });
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int64_t i = 0; i < n; ++i) {
     keys[i] = v[i].first;
     if (vals) vals[i] = v[i].second;
   }
+  return dt;
 }
 
 template <typename K>
@@ -451,6 +457,15 @@ void oracle_stable_sort(int kt, void* keys, uint32_t* vals, int64_t n, int great
     typedef std::remove_pointer_t<decltype(tag)> K;
     stable_sort_kv((K*)keys, vals, n, greater);
   });
+}
+// The same sort; returns the seconds std::stable_sort took (the CPU baseline).
+double oracle_stable_sort_timed(int kt, void* keys, uint32_t* vals, int64_t n, int greater) {
+  double dt = 0;
+  dispatch_key(kt, [&](auto* tag) {
+    typedef std::remove_pointer_t<decltype(tag)> K;
+    dt = stable_sort_kv((K*)keys, vals, n, greater);
+  });
+  return dt;
 }
 
 // One std::stable_sort per segment [segs[i], segs[i+1]) (last ends at n),
