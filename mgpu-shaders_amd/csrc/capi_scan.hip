@@ -4,23 +4,23 @@
 
 using namespace mgpu::hip;
 
-constexpr int SNT = scan_tuning::nt, SVT = scan_tuning::vt;
-
 #define MGPU_DEF_SCAN(T, TT)                                                          \
   extern "C" mgpu_status_t mgpu_scan_exclusive_##T(void* aux, size_t* aux_size,       \
                                                    mgpu_stream_t stream, TT* data,    \
                                                    int64_t count, TT init) {          \
     if (!aux_size) return MGPU_INVALID_ARG;                                           \
-    return scan<SNT, SVT>(aux, *aux_size, (hipStream_t)stream, (const TT*)data, data, \
-                          count, MGPU_SCAN_EXCLUSIVE, init);                          \
+    constexpr int NT = scan_tuning<TT>::nt, VT = scan_tuning<TT>::vt;                 \
+    return scan<NT, VT>(aux, *aux_size, (hipStream_t)stream, (const TT*)data, data,   \
+                        count, MGPU_SCAN_EXCLUSIVE, init);                            \
   }                                                                                   \
   extern "C" mgpu_status_t mgpu_scan_##T(void* aux, size_t* aux_size,                 \
                                          mgpu_stream_t stream, const TT* input,       \
                                          TT* output, int64_t count,                   \
                                          mgpu_scan_type_t type, TT init) {            \
     if (!aux_size) return MGPU_INVALID_ARG;                                           \
-    return scan<SNT, SVT>(aux, *aux_size, (hipStream_t)stream, input, output, count,  \
-                          type, init);                                                \
+    constexpr int NT = scan_tuning<TT>::nt, VT = scan_tuning<TT>::vt;                 \
+    return scan<NT, VT>(aux, *aux_size, (hipStream_t)stream, input, output, count,    \
+                        type, init);                                                  \
   }                                                                                   \
   extern "C" mgpu_status_t mgpu_reduce_##T(void* aux, size_t* aux_size,               \
                                            mgpu_stream_t stream, const TT* input,     \

@@ -11,7 +11,7 @@ using namespace mgpu::hip;
 
 namespace {
 
-constexpr int SNT = scan_tuning::nt, SVT = scan_tuning::vt;
+constexpr int SNT = scan_tuning<int32_t>::nt, SVT = scan_tuning<int32_t>::vt;
 // One 64-bit status word per 20480-element tile: 1 MiB covers 131072 tiles,
 // i.e. up to 2,684,354,560 elements per call.
 constexpr int64_t noalloc_max_tiles = 131072;

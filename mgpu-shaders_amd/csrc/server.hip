@@ -113,7 +113,7 @@ size_t request_bytes(const mgpu_request_t& r) {
       if (r.key_type == MGPU_KEY_U64)
         scan64<uint64_t>(nullptr, aux, nullptr, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
       else
-        scan<scan_tuning::nt, scan_tuning::vt>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
+        scan<scan_tuning<uint32_t>::nt, scan_tuning<uint32_t>::vt>(nullptr, aux, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
                       MGPU_SCAN_EXCLUSIVE, 0u);
       return 2 * aux_round(n * kb) + aux_round(aux);
   }
@@ -155,7 +155,7 @@ size_t caller_aux_bytes(const mgpu_request_t& r, const mgpu_request_buffers_t& b
       if (r.key_type == MGPU_KEY_U64)
         scan64<uint64_t>(nullptr, aux, nullptr, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
       else
-        scan<scan_tuning::nt, scan_tuning::vt>(nullptr, aux, nullptr, (const uint32_t*)nullptr,
+        scan<scan_tuning<uint32_t>::nt, scan_tuning<uint32_t>::vt>(nullptr, aux, nullptr, (const uint32_t*)nullptr,
                                                (uint32_t*)nullptr, n, MGPU_SCAN_EXCLUSIVE, 0u);
       break;
   }
@@ -224,15 +224,15 @@ mgpu_status_t enqueue_caller(rec_t* r, hipStream_t s) {
           return scan64<uint64_t>(r->aux, c, s, (const uint64_t*)b.in_a, (uint64_t*)b.out, n,
                                   (mgpu_scan_type_t)b.scan_type, 0ull);
         case MGPU_KEY_F32:
-          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, c, s, (const float*)b.in_a,
+          return scan<scan_tuning<float>::nt, scan_tuning<float>::vt>(r->aux, c, s, (const float*)b.in_a,
                                                         (float*)b.out, n,
                                                         (mgpu_scan_type_t)b.scan_type, 0.0f);
         case MGPU_KEY_I32:
-          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, c, s, (const int32_t*)b.in_a,
+          return scan<scan_tuning<uint32_t>::nt, scan_tuning<uint32_t>::vt>(r->aux, c, s, (const int32_t*)b.in_a,
                                                         (int32_t*)b.out, n,
                                                         (mgpu_scan_type_t)b.scan_type, 0);
         default:
-          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, c, s, (const uint32_t*)b.in_a,
+          return scan<scan_tuning<uint32_t>::nt, scan_tuning<uint32_t>::vt>(r->aux, c, s, (const uint32_t*)b.in_a,
                                                         (uint32_t*)b.out, n,
                                                         (mgpu_scan_type_t)b.scan_type, 0u);
       }
@@ -323,7 +323,7 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
       if (q.key_type == MGPU_KEY_U64)
         scan64<uint64_t>(nullptr, aux, s, nullptr, nullptr, n, MGPU_SCAN_EXCLUSIVE, 0ull);
       else
-        scan<scan_tuning::nt, scan_tuning::vt>(nullptr, aux, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
+        scan<scan_tuning<uint32_t>::nt, scan_tuning<uint32_t>::vt>(nullptr, aux, s, (const uint32_t*)nullptr, (uint32_t*)nullptr, n,
                       MGPU_SCAN_EXCLUSIVE, 0u);
       if (!ok(alloc(r, &r->in_a, n * kb, s)) || !ok(alloc(r, &r->out, n * kb, s)) ||
           !ok(alloc(r, &r->aux, aux, s)))
@@ -336,10 +336,10 @@ mgpu_status_t enqueue(rec_t* r, hipStream_t s) {
           return scan64<uint64_t>(r->aux, cap, s, (const uint64_t*)r->in_a, (uint64_t*)r->out, n,
                                   MGPU_SCAN_EXCLUSIVE, 0ull);
         case MGPU_KEY_F32:
-          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, cap, s, (const float*)r->in_a, (float*)r->out, n,
+          return scan<scan_tuning<float>::nt, scan_tuning<float>::vt>(r->aux, cap, s, (const float*)r->in_a, (float*)r->out, n,
                                MGPU_SCAN_INCLUSIVE, 0.0f);
         default:
-          return scan<scan_tuning::nt, scan_tuning::vt>(r->aux, cap, s, (const uint32_t*)r->in_a, (uint32_t*)r->out, n,
+          return scan<scan_tuning<uint32_t>::nt, scan_tuning<uint32_t>::vt>(r->aux, cap, s, (const uint32_t*)r->in_a, (uint32_t*)r->out, n,
                                MGPU_SCAN_EXCLUSIVE, 0u);
       }
     }

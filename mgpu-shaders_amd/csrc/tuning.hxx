@@ -14,10 +14,16 @@ template <typename key_t, int bytes = sizeof(key_t)> struct merge_tuning { stati
 template <typename key_t> struct merge_tuning<key_t, 8> { static constexpr int nt = 512, vt = 15; };
 template <typename key_t, int bytes = sizeof(key_t)> struct mergesort_tuning { static constexpr int nt = 512, vt = 19; };
 template <typename key_t> struct mergesort_tuning<key_t, 8> { static constexpr int nt = 512, vt = 13; };
-// Scan: 256 x 80 (20480-element tiles).  Larger tiles mean fewer look-back
-// steps per byte; 2^29 int32 exclusive: 0.775 ms at 256 x 80 vs 1.19 ms at
-// 256 x 16 on MI355X (float32 inclusive 0.822 vs 1.26 ms).
-struct scan_tuning { static constexpr int nt = 256, vt = 80; };
+// Scan: 256 x 80 (20480-element tiles) for 32-bit integers, 256 x 96 for
+// float32.  Larger tiles mean fewer look-back steps per byte; 2^29 int32
+// exclusive: 0.775 ms at 256 x 80 vs 1.19 ms at 256 x 16 on MI355X, 0.781 at
+// 256 x 96, 0.823 at 256 x 128, 0.816 at 512 x 48, 0.830 at 1024 x 32;
+// float32 inclusive: 0.788 ms at 256 x 96 vs 0.818 at 256 x 80, 0.846 at
+// 256 x 112, 0.842 at 512 x 64.  Forcing four waves per SIMD (128 VGPRs)
+// was slower for both (int32 0.796, float32 0.820 with spills).
+template <typename T> struct scan_tuning {
+  static constexpr int nt = 256, vt = std::is_same<T, float>::value ? 96 : 80;
+};
 // Segmented scan: integers 256 x 48, float32 256 x 32.  2^29 int32 over
 // 65536 segments, exclusive: 0.843 ms at 256 x 48, 0.890 at 256 x 32, 0.854
 // at 256 x 64 (inclusive 0.889 / 0.921 / 1.022).  The float32 kernels carry

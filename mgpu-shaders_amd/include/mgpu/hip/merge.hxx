@@ -58,8 +58,10 @@ __device__ inline int64_t merge_path_global(const key_t* a, int64_t a_count, con
 
 // Serial merge of vt outputs starting at (a_begin, b_begin) of LDS keys, with
 // the range checks of merge_predicate (cta_merge.hxx:41-50).  keys_shared
-// must be readable one element past the ranges.
-template <int VT, typename key_t, typename comp_t>
+// must be readable one element past the ranges.  With checked = false the
+// caller guarantees that neither range runs out within the VT steps (both
+// hold at least VT keys), and the range tests are dropped.
+template <int VT, bool checked = true, typename key_t, typename comp_t>
 __device__ inline void serial_merge(const key_t* keys_shared, int a_begin, int a_end,
                                     int b_begin, int b_end, key_t (&keys)[VT],
                                     int (&indices)[VT], comp_t comp) {
@@ -67,17 +69,31 @@ __device__ inline void serial_merge(const key_t* keys_shared, int a_begin, int a
   key_t b_key = keys_shared[b_begin];
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
-    bool p;
-    if (a_begin >= a_end) p = false;
-    else if (b_begin >= b_end) p = true;
-    else p = !comp(b_key, a_key);
-    int index = p ? a_begin : b_begin;
+    const bool p = checked ? (a_begin < a_end && (b_begin >= b_end || !comp(b_key, a_key)))
+                           : !comp(b_key, a_key);
+    const int index = p ? a_begin : b_begin;
     keys[i] = p ? a_key : b_key;
     indices[i] = index;
-    key_t c_key = keys_shared[index + 1];
-    if (p) { a_key = c_key; a_begin = index + 1; }
-    else { b_key = c_key; b_begin = index + 1; }
+    const key_t c_key = keys_shared[index + 1];
+    a_begin = p ? index + 1 : a_begin;
+    b_begin = p ? b_begin : index + 1;
+    a_key = p ? c_key : a_key;
+    b_key = p ? b_key : c_key;
   }
+}
+
+// serial_merge with the range tests dropped when every lane of the wave has
+// at least VT keys left in both ranges (a wave-uniform choice, so the wave
+// runs one of the two loops, never both).
+template <int VT, typename key_t, typename comp_t>
+__device__ inline void serial_merge_wave(const key_t* keys_shared, int a_begin, int a_end,
+                                         int b_begin, int b_end, key_t (&keys)[VT],
+                                         int (&indices)[VT], comp_t comp) {
+  const bool roomy = a_end - a_begin >= VT && b_end - b_begin >= VT;
+  if (__all(roomy))
+    serial_merge<VT, false>(keys_shared, a_begin, a_end, b_begin, b_end, keys, indices, comp);
+  else
+    serial_merge<VT, true>(keys_shared, a_begin, a_end, b_begin, b_end, keys, indices, comp);
 }
 
 // Partition kernel: mp[i] = merge_path(a, b, min(i * spacing, a_count + b_count))
@@ -181,7 +197,7 @@ __device__ inline void cta_merge_tile(
   int mp = merge_path_lds(s.keys, a_cnt, s.keys + a_cnt, b_cnt, diag, comp);
   key_t keys[VT];
   int indices[VT];
-  serial_merge<VT>(s.keys, mp, a_cnt, a_cnt + diag - mp, total, keys, indices, comp);
+  serial_merge_wave<VT>(s.keys, mp, a_cnt, a_cnt + diag - mp, total, keys, indices, comp);
   val_t vals[VT];
   if constexpr (has_values) {
 #pragma unroll

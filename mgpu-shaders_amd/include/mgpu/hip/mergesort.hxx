@@ -68,6 +68,9 @@ struct blocksort_storage_t {
 
 // Stable odd-even transposition sort of VT registers; bit j of `oor` marks
 // slot j as out of range (it never moves and nothing swaps with it).
+// Each compare-exchange is written as selects (v_cndmask), not as a
+// conditional swap: the compiler turns a conditional swap of several
+// registers into an exec-masked branch with copies on both sides.
 template <int VT, bool has_values, bool masked = true, typename key_t, typename val_t,
           typename comp_t>
 __device__ inline void odd_even_sort(key_t (&k)[VT], val_t (&v)[VT], uint32_t oor, comp_t comp) {
@@ -75,9 +78,16 @@ __device__ inline void odd_even_sort(key_t (&k)[VT], val_t (&v)[VT], uint32_t oo
   for (int I = 0; I < VT; ++I) {
 #pragma unroll
     for (int i = 1 & I; i < VT - 1; i += 2) {
-      if ((!masked || !((oor >> (i + 1)) & 1)) && comp(k[i + 1], k[i])) {
-        key_t t = k[i]; k[i] = k[i + 1]; k[i + 1] = t;
-        if constexpr (has_values) { val_t u = v[i]; v[i] = v[i + 1]; v[i + 1] = u; }
+      const bool swap = (!masked || !((oor >> (i + 1)) & 1)) && comp(k[i + 1], k[i]);
+      const key_t lo = swap ? k[i + 1] : k[i];
+      const key_t hi = swap ? k[i] : k[i + 1];
+      k[i] = lo;
+      k[i + 1] = hi;
+      if constexpr (has_values) {
+        const val_t vlo = swap ? v[i + 1] : v[i];
+        const val_t vhi = swap ? v[i] : v[i + 1];
+        v[i] = vlo;
+        v[i + 1] = vhi;
       }
     }
   }

@@ -11,7 +11,8 @@ from gpu_util import banner, dev_equal, gen_dev, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
-NV = 256 * 80  # elements per tile of the library's scan
+NV = 256 * 80  # elements per tile of the library's 32-bit integer scan
+F32_NV = 256 * 96  # elements per tile of its float32 scan
 SIZES = [1, 2, 3, 4, 5, 63, 64, 65, 255, 256, 257, 4095, 4096, 4097, NV - 1, NV, NV + 1,
          3 * NV + 7, 64 * NV + 1, 1 << 20,
          (1 << 22) + 13]
@@ -78,7 +79,8 @@ def test_vk_scan_known_answer():
     np.testing.assert_array_equal(to_host(d).astype(np.int64), exp)
 
 
-@pytest.mark.parametrize("n", [1, 4095, 4097, 1 << 20, (1 << 22) + 13])
+@pytest.mark.parametrize("n", [1, 4095, 4097, F32_NV - 1, F32_NV, F32_NV + 1, 3 * F32_NV + 7,
+                               1 << 20, (1 << 22) + 13])
 @pytest.mark.parametrize("inclusive", [True, False])
 def test_scan_f32_error_bound(n, inclusive):
     banner("scan_f32", n, inclusive)
