@@ -32,7 +32,7 @@ The JSON line also carries:
   serving       the request-serving engine at saturation and three loads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-secondary]
-                       [--no-cpu] [--no-serving] [--no-segsort] [--no-chart]
+                       [--no-cpu] [--no-serving] [--no-segsort] [--no-segsort-loop] [--no-chart]
                        [--no-vendor] [--cpu-log2 L]
 """
 import argparse
@@ -304,12 +304,66 @@ def secondary(torch, mgpu_hip, steps, warmup, chart=True):
     return out
 
 
-def bench_segsort(torch, mgpu_hip, steps, warmup):
+def segsort_loop_baseline(torch, mgpu_hip, k, v, restore, starts, lens, batched_k, batched_v):
+    """One plain mergesort call per segment (segments of length <= 1 are
+    skipped), through the C ABI with a pre-sized aux buffer: as Python calls,
+    and recorded once into a graph and replayed (the GPU-side cost of one
+    launch sequence per segment, without the host dispatch of each call).
+    Both results are compared with the batched sort's on the device."""
+    import ctypes
+    lib = mgpu_hip.load()
+    fn = lib.mgpu_mergesort_kv_u32_u32
+    big = int(lens.max())
+    aux_loop = mgpu_hip.mergesort_kv(k[:big], v[:big])
+    cap = ctypes.c_size_t(aux_loop.numel())
+    auxp = ctypes.c_void_p(aux_loop.data_ptr())
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    kb, vb = k.data_ptr(), v.data_ptr()
+    work = [(ctypes.c_void_p(kb + 4 * int(s)), ctypes.c_void_p(vb + 4 * int(s)), int(l))
+            for s, l in zip(starts, lens) if l > 1]
+    log("segsort loop baseline: %d plain sorts" % len(work))
+    restore()
+    for kp, vp, l in work[:200]:
+        fn(auxp, ctypes.byref(cap), sp, kp, vp, l, 0)
+    restore()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for kp, vp, l in work:
+        st = fn(auxp, ctypes.byref(cap), sp, kp, vp, l, 0)
+        if st:
+            raise RuntimeError("plain mergesort failed: %d" % st)
+    torch.cuda.synchronize()
+    ms_loop = (time.perf_counter() - t0) * 1e3
+    same = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
+                torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
+    log("segsort loop baseline: %.1f ms as calls; recording the graph" % ms_loop)
+
+    lstream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    lsp = ctypes.c_void_p(lstream.cuda_stream)
+    with torch.cuda.stream(lstream):
+        lg = mgpu_hip.Graph(lstream)
+        with lg:
+            for kp, vp, l in work:
+                st = fn(auxp, ctypes.byref(cap), lsp, kp, vp, l, 0)
+                if st:
+                    raise RuntimeError("plain mergesort failed: %d" % st)
+        ms_loop_graph, _ = timed_steps(torch, 3, 1, restore, lg.launch)
+    torch.cuda.synchronize()
+    same_graph = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
+                      torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
+    del lg
+    return {"ms_loop_of_plain_sorts": ms_loop, "plain_sort_calls": len(work),
+            "ms_loop_of_plain_sorts_graph": ms_loop_graph,
+            "outputs_identical": same and same_graph}
+
+
+def bench_segsort(torch, mgpu_hip, steps, warmup, loop_baseline=True):
     """65536 lognormal segments (median 256, sigma 1.5, clamped to [1, 2^20]),
     uint32 keys + uint32 values: batched segmented sort (eager and replayed
     as one graph) against a loop of plain mergesort calls, one per segment,
-    on the same GPU; outputs compared on the device."""
-    import ctypes
+    on the same GPU; outputs compared on the device.  loop_baseline=False
+    leaves the loop out (its 65k calls crawl under a counter profiler)."""
     import numpy as np
     from mgpu_hip import workloads
     lens = workloads.lognormal_lengths(65536, 256.0, 1.5, 1, 1 << 20, seed=0)
@@ -345,52 +399,13 @@ def bench_segsort(torch, mgpu_hip, steps, warmup):
         ms_graph, _ = timed_steps(torch, steps, warmup, restore, g.launch)
     torch.cuda.synchronize()
     batched_k, batched_v = k.clone(), v.clone()
-
-    # Baseline: one plain mergesort call per segment (segments of length <= 1
-    # are skipped), through the C ABI with a pre-sized aux buffer.
-    lib = mgpu_hip.load()
-    fn = lib.mgpu_mergesort_kv_u32_u32
-    big = int(lens.max())
-    aux_loop = mgpu_hip.mergesort_kv(k[:big], v[:big])
-    cap = ctypes.c_size_t(aux_loop.numel())
-    auxp = ctypes.c_void_p(aux_loop.data_ptr())
-    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    kb, vb = k.data_ptr(), v.data_ptr()
-    work = [(ctypes.c_void_p(kb + 4 * int(s)), ctypes.c_void_p(vb + 4 * int(s)), int(l))
-            for s, l in zip(starts, lens) if l > 1]
-    restore()
-    for kp, vp, l in work[:200]:
-        fn(auxp, ctypes.byref(cap), sp, kp, vp, l, 0)
-    restore()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for kp, vp, l in work:
-        st = fn(auxp, ctypes.byref(cap), sp, kp, vp, l, 0)
-        if st:
-            raise RuntimeError("plain mergesort failed: %d" % st)
-    torch.cuda.synchronize()
-    ms_loop = (time.perf_counter() - t0) * 1e3
-    same = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
-                torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
-
-    # The same loop of plain sorts recorded once into a graph and replayed:
-    # the GPU-side cost of one launch sequence per segment, without the host
-    # dispatch of each call.
-    lstream = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    lsp = ctypes.c_void_p(lstream.cuda_stream)
-    with torch.cuda.stream(lstream):
-        lg = mgpu_hip.Graph(lstream)
-        with lg:
-            for kp, vp, l in work:
-                st = fn(auxp, ctypes.byref(cap), lsp, kp, vp, l, 0)
-                if st:
-                    raise RuntimeError("plain mergesort failed: %d" % st)
-        ms_loop_graph, _ = timed_steps(torch, 3, 1, restore, lg.launch)
-    torch.cuda.synchronize()
-    same_graph = bool(torch.equal(k.view(torch.int32), batched_k.view(torch.int32)) and
-                      torch.equal(v.view(torch.int32), batched_v.view(torch.int32)))
-    del lg
+    log("segsort batched: eager %.4f ms, graph %.4f ms" % (ms_eager, ms_graph))
+    if loop_baseline:
+        loop = segsort_loop_baseline(torch, mgpu_hip, k, v, restore, starts, lens,
+                                     batched_k, batched_v)
+    else:
+        loop = {"ms_loop_of_plain_sorts": None, "plain_sort_calls": None,
+                "ms_loop_of_plain_sorts_graph": None, "outputs_identical": None}
 
     # Algorithmic bytes: 16 B per key-value pair for every pass actually run:
     # one read and one write of the segments the blocksort buckets hold, four
@@ -409,11 +424,11 @@ def bench_segsort(torch, mgpu_hip, steps, warmup):
                         "achieved": alg_bytes / (ms_graph / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s",
                         "frac": alg_bytes / (ms_graph / 1e3) / 1e9 / HBM_PEAK_GBS},
-           "ms_loop_of_plain_sorts": ms_loop, "plain_sort_calls": len(work),
-           "ms_loop_of_plain_sorts_graph": ms_loop_graph,
-           "speedup_vs_loop": ms_loop / ms_graph,
-           "speedup_vs_loop_graph": ms_loop_graph / ms_graph,
-           "outputs_identical": same and same_graph,
+           **loop,
+           "speedup_vs_loop": (loop["ms_loop_of_plain_sorts"] / ms_graph
+                               if loop_baseline else None),
+           "speedup_vs_loop_graph": (loop["ms_loop_of_plain_sorts_graph"] / ms_graph
+                                     if loop_baseline else None),
            "tuning": {"cap": tuning["cap"], "shape": [list(shapes[s]) for s in tuning["shape"]]},
            "kernels_ms": {k2: round(v2[1], 4) for k2, v2 in kt.items()}}
     log("segsort", {k2: out[k2] for k2 in ("ms_batched_eager", "ms_batched_graph",
@@ -735,6 +750,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-segsort", action="store_true")
+    ap.add_argument("--no-segsort-loop", action="store_true",
+                    help="leave out the segmented sort's loop-of-plain-sorts comparison")
     ap.add_argument("--no-chart", action="store_true",
                     help="skip the key-only radix sizes of the reference chart")
     ap.add_argument("--no-vendor", action="store_true",
@@ -812,7 +829,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_secondary:
         if not args.no_segsort:
-            result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2)
+            result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2,
+                                             loop_baseline=not args.no_segsort_loop)
             torch.cuda.empty_cache()
         result["secondary"] = secondary(torch, mgpu_hip, max(3, min(args.steps, 10)), 2,
                                         chart=not args.no_chart)
