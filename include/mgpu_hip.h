@@ -79,7 +79,8 @@ mgpu_status_t mgpu_init(void);
 /* ------------------------------------------------------------------------ */
 /* Mergesort: stable, in place.  Replaces vk::mergesort_keys / mergesort_kv /
  * mergesort_indices (inc/mgpu/vk/mergesort.hxx:86-93, 10-84, 95-103).
- * aux: ping-pong keys (+ values) and merge-path partitions.
+ * aux: ping-pong keys (+ values), merge-path partitions and, for four-way
+ * passes, one sample per tile.
  * Key types: u32, i32, u64, f32, i64, f64.  Values: u32.  indices:
  * vals[i] := i before the sort (inc/mgpu/common/kernel_mergesort.hxx:43-49).
  * count < 2^31.                                                             */
@@ -98,6 +99,15 @@ MGPU_DECL_MERGESORT(u64, uint64_t)
 MGPU_DECL_MERGESORT(f32, float)
 MGPU_DECL_MERGESORT(i64, int64_t)
 MGPU_DECL_MERGESORT(f64, double)
+/* Merge pass structure (process-wide): 0 = automatic, 2 = two-way passes
+ * only (the reference's structure: one merge-path pass per level), 4 =
+ * four-way passes (two levels per pass; one two-way pass first when the
+ * number of levels is odd) at every size.  Automatic takes four-way passes
+ * once a pass moves 256 MB of keys and values (2^28 u32 pairs: 8 global
+ * passes instead of 15).  Results are identical; the sizing call returns
+ * enough aux for either.  No counterpart in the reference.                  */
+void mgpu_set_mergesort_passes(int ways);
+int mgpu_mergesort_passes(void);
 
 /* ------------------------------------------------------------------------ */
 /* Two-way stable merge (ties take A first), out of place into c.

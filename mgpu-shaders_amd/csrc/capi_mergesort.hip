@@ -54,3 +54,8 @@ MGPU_DEF_MERGESORT(u64, uint64_t)
 MGPU_DEF_MERGESORT(f32, float)
 MGPU_DEF_MERGESORT(i64, int64_t)
 MGPU_DEF_MERGESORT(f64, double)
+
+extern "C" void mgpu_set_mergesort_passes(int ways) {
+  mergesort_passes_setting().store(ways == 2 || ways == 4 ? ways : 0);
+}
+extern "C" int mgpu_mergesort_passes(void) { return mergesort_passes_setting().load(); }

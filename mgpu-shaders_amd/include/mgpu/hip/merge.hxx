@@ -41,6 +41,218 @@ __host__ __device__ inline index_t merge_path_search(const key_t* a, index_t a_c
   return begin;
 }
 
+// Split of the first `diag` outputs of the stable merge of four sorted runs
+// r0..r3 (ties go to the earlier run): split[k] = the number taken from r_k.
+// The merge is merge(X, Y) with X = merge(r0, r1) and Y = merge(r2, r3), so
+// the search is a merge path over X and Y whose probes X[m] and Y[n] are
+// themselves merge paths over (r0, r1) and (r2, r3).  a(m), the number of r0
+// keys among X's first m, is non-decreasing in m and m - a(m) is too, so
+// every probe evaluated so far bounds the inner search of the next: after
+// probes at ml < mh, a(m) for ml <= m <= mh lies in
+// [max(a(ml), a(mh) - (mh - m)), min(a(mh), a(ml) + (m - ml))]; the same holds
+// for Y with b(n), the number of r2 keys among Y's first n.  The two inner
+// searches of one probe run interleaved.
+//
+// merge4_state_t carries the search range [lo, hi] of the answer i (the
+// number of X keys among the first diag) and the probes that bound a() and
+// b(): any known values a(xlm) = xla, a(xhm) = xha with xlm <= i <= xhm and
+// b(yln) = ylb, b(yhn) = yhb with yln <= diag - i <= yhn.
+struct merge4_state_t {
+  int64_t lo, hi;
+  int64_t xlm, xla, xhm, xha;
+  int64_t yln, ylb, yhn, yhb;
+};
+
+// The state that knows only the run lengths.
+__host__ __device__ inline merge4_state_t merge4_initial(int64_t n0, int64_t n1, int64_t n2,
+                                                         int64_t n3, int64_t diag) {
+  const int64_t nx = n0 + n1, ny = n2 + n3;
+  return merge4_state_t{max_<int64_t>(0, diag - ny), min_(diag, nx), 0, 0, nx, n0, 0, 0, ny, n2};
+}
+
+// Probes per search round.  With P > 1 a round is a (P+1)-ary search whose
+// P probes are independent loads, log_{P+1}(w) dependent rounds instead of
+// log2(w); on MI355X the partition kernels are bound by the number of
+// scattered cache lines their probes fetch, not by the length of the chain,
+// and P = 3 or 7 made them slower (2^28 pairs: partition 0.13 -> 0.20 ms,
+// samples 0.05 -> 0.08 ms at P = 3), so the searches are binary.
+constexpr int merge4_probes = 1;
+
+template <typename key_t, typename comp_t, int P = merge4_probes>
+__host__ __device__ inline void merge4_search(const key_t* r0, int64_t n0, const key_t* r1,
+                                              int64_t n1, const key_t* r2, int64_t n2,
+                                              const key_t* r3, int64_t n3, int64_t diag,
+                                              comp_t comp, merge4_state_t st, int64_t (&split)[4]) {
+  // Merge paths a(m[p]) of (r0, r1) and b(n[p]) of (r2, r3) for Q probes at
+  // once, within the probe bounds of st: 2Q binary searches, interleaved.
+  auto inner = [&](auto const (&m)[P], auto const (&n)[P], int q, int64_t (&a)[P],
+                   int64_t (&b)[P]) {
+    int64_t xb[P], xe[P], yb[P], ye[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      xb[p] = max_(max_(st.xla, st.xha - (st.xhm - m[p])), max_<int64_t>(0, m[p] - n1));
+      xe[p] = min_(min_(st.xha, st.xla + (m[p] - st.xlm)), min_(m[p], n0));
+      yb[p] = max_(max_(st.ylb, st.yhb - (st.yhn - n[p])), max_<int64_t>(0, n[p] - n3));
+      ye[p] = min_(min_(st.yhb, st.ylb + (n[p] - st.yln)), min_(n[p], n2));
+      if (p >= q) xe[p] = xb[p], ye[p] = yb[p];
+    }
+    bool busy = true;
+    while (busy) {
+      busy = false;
+      key_t xa[P], xk[P], ya[P], yk[P];
+      int64_t xm[P], ym[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        xm[p] = (xb[p] + xe[p]) >> 1;
+        ym[p] = (yb[p] + ye[p]) >> 1;
+        if (xb[p] < xe[p]) {
+          xa[p] = r0[xm[p]];
+          xk[p] = r1[m[p] - 1 - xm[p]];
+        }
+        if (yb[p] < ye[p]) {
+          ya[p] = r2[ym[p]];
+          yk[p] = r3[n[p] - 1 - ym[p]];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        if (xb[p] < xe[p]) {
+          if (!comp(xk[p], xa[p])) xb[p] = xm[p] + 1;
+          else xe[p] = xm[p];
+        }
+        if (yb[p] < ye[p]) {
+          if (!comp(yk[p], ya[p])) yb[p] = ym[p] + 1;
+          else ye[p] = ym[p];
+        }
+        busy |= xb[p] < xe[p] || yb[p] < ye[p];
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      a[p] = xb[p];
+      b[p] = yb[p];
+    }
+  };
+  int64_t m[P], n[P], a[P], b[P];
+  while (st.lo < st.hi) {
+    const int64_t w = st.hi - st.lo;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      m[p] = st.lo + w * (p + 1) / (P + 1);  // lo <= m < hi, so m < nx
+      n[p] = diag - 1 - m[p];                // and 0 <= n < ny
+    }
+    inner(m, n, P, a, b);
+    bool before[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      // X[m] and Y[n]: the next output of each inner merge.
+      const bool xr0 = a[p] < n0 && (m[p] - a[p] >= n1 || !comp(r1[m[p] - a[p]], r0[a[p]]));
+      const bool yr2 = b[p] < n2 && (n[p] - b[p] >= n3 || !comp(r3[n[p] - b[p]], r2[b[p]]));
+      const key_t xkey = xr0 ? r0[a[p]] : r1[m[p] - a[p]];
+      const key_t ykey = yr2 ? r2[b[p]] : r3[n[p] - b[p]];
+      before[p] = !comp(ykey, xkey);  // X[m] precedes Y[n]: more than m outputs from X
+    }
+    // Probes ascend in m and `before` is true then false along them: the last
+    // true probe raises lo, the first false one lowers hi.
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (before[p]) {
+        if (m[p] + 1 > st.lo) {
+          st.lo = m[p] + 1;
+          st.xlm = m[p];
+          st.xla = a[p];
+          st.yhn = n[p];
+          st.yhb = b[p];
+        }
+      } else if (m[p] < st.hi) {
+        st.hi = m[p];
+        st.xhm = m[p];
+        st.xha = a[p];
+        st.yln = n[p];
+        st.ylb = b[p];
+      }
+    }
+  }
+  m[0] = st.lo;
+  n[0] = diag - st.lo;
+  inner(m, n, 1, a, b);
+  split[0] = a[0];
+  split[1] = st.lo - a[0];
+  split[2] = b[0];
+  split[3] = diag - st.lo - b[0];
+}
+
+template <typename key_t, typename comp_t>
+__host__ __device__ inline void merge4_path(const key_t* r0, int64_t n0, const key_t* r1,
+                                            int64_t n1, const key_t* r2, int64_t n2,
+                                            const key_t* r3, int64_t n3, int64_t diag,
+                                            comp_t comp, int64_t (&split)[4]) {
+  merge4_search(r0, n0, r1, n1, r2, n2, r3, n3, diag, comp, merge4_initial(n0, n1, n2, n3, diag),
+                split);
+}
+
+// Smallest i in [begin, end) at which pred (true, then false, along the
+// range) is false, with P probes per round; two such searches run
+// interleaved in kary_search2.
+template <int P = merge4_probes, typename pred_t>
+__host__ __device__ inline int64_t kary_search(int64_t begin, int64_t end, pred_t pred) {
+  while (begin < end) {
+    const int64_t w = end - begin;
+    int64_t mid[P];
+    bool t[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) mid[p] = begin + w * (p + 1) / (P + 1);
+#pragma unroll
+    for (int p = 0; p < P; ++p) t[p] = pred(mid[p]);
+    int64_t nb = begin, ne = end;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (t[p]) nb = max_(nb, mid[p] + 1);
+      else ne = min_(ne, mid[p]);
+    }
+    begin = nb;
+    end = ne;
+  }
+  return begin;
+}
+
+template <int P = merge4_probes, typename pred1_t, typename pred2_t>
+__host__ __device__ inline void kary_search2(int64_t& b1, int64_t e1, pred1_t pred1, int64_t& b2,
+                                             int64_t e2, pred2_t pred2) {
+  while (b1 < e1 || b2 < e2) {
+    const bool s1 = b1 < e1, s2 = b2 < e2;
+    const int64_t w1 = e1 - b1, w2 = e2 - b2;
+    int64_t m1[P], m2[P];
+    bool t1[P], t2[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      m1[p] = b1 + w1 * (p + 1) / (P + 1);
+      m2[p] = b2 + w2 * (p + 1) / (P + 1);
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (s1) t1[p] = pred1(m1[p]);
+      if (s2) t2[p] = pred2(m2[p]);
+    }
+    int64_t nb1 = b1, ne1 = e1, nb2 = b2, ne2 = e2;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      if (s1) {
+        if (t1[p]) nb1 = max_(nb1, m1[p] + 1);
+        else ne1 = min_(ne1, m1[p]);
+      }
+      if (s2) {
+        if (t2[p]) nb2 = max_(nb2, m2[p] + 1);
+        else ne2 = min_(ne2, m2[p]);
+      }
+    }
+    b1 = nb1;
+    e1 = ne1;
+    b2 = nb2;
+    e2 = ne2;
+  }
+}
+
 // Per-thread merge path (binary search), used in LDS.
 template <typename key_t, typename comp_t>
 __device__ inline int merge_path_lds(const key_t* a, int a_count, const key_t* b,

@@ -119,6 +119,10 @@ def load():
     lib.mgpu_generate.argtypes = [vp, vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
     lib.mgpu_set_radix_algorithm.restype = None
     lib.mgpu_set_radix_algorithm.argtypes = [c_int]
+    lib.mgpu_set_mergesort_passes.restype = None
+    lib.mgpu_set_mergesort_passes.argtypes = [c_int]
+    lib.mgpu_mergesort_passes.restype = c_int
+    lib.mgpu_mergesort_passes.argtypes = []
     lib.mgpu_radix_algorithm.restype = c_int
     lib.mgpu_set_lookback_spin_limit.restype = None
     lib.mgpu_set_lookback_spin_limit.argtypes = [ctypes.c_uint32]
@@ -519,6 +523,18 @@ def set_radix_algorithm(algorithm):
 
 def radix_algorithm():
     return int(load().mgpu_radix_algorithm())
+
+
+MERGESORT_AUTO, MERGESORT_TWO_WAY, MERGESORT_FOUR_WAY = 0, 2, 4
+
+
+def set_mergesort_passes(ways):
+    """0 = automatic, 2 = two-way merge passes only, 4 = four-way passes."""
+    load().mgpu_set_mergesort_passes(int(ways))
+
+
+def mergesort_passes():
+    return int(load().mgpu_mergesort_passes())
 
 
 def set_lookback_spin_limit(limit):

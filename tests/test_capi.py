@@ -56,6 +56,25 @@ def test_radix_algorithm_setting():
         mgpu_hip.set_radix_algorithm(old)
 
 
+def test_mergesort_passes_setting():
+    """0 automatic, 2 two-way, 4 four-way; anything else falls back to
+    automatic.  Sizing does not depend on the setting."""
+    old = mgpu_hip.mergesort_passes()
+    one = ctypes.c_void_p(1)
+    try:
+        sizes = []
+        for w in (0, 2, 4):
+            mgpu_hip.set_mergesort_passes(w)
+            assert mgpu_hip.mergesort_passes() == w
+            sizes.append(_size(mgpu_hip.load().mgpu_mergesort_kv_u32_u32, one, one, 1 << 26, 0))
+        assert sizes[0] == sizes[1] == sizes[2]
+        for bad in (-1, 1, 3, 8):
+            mgpu_hip.set_mergesort_passes(bad)
+            assert mgpu_hip.mergesort_passes() == 0
+    finally:
+        mgpu_hip.set_mergesort_passes(old)
+
+
 def _size(fn, *args):
     size = ctypes.c_size_t(0)
     st = fn(None, ctypes.byref(size), None, *args)
@@ -66,16 +85,30 @@ def r256(x):
     return (x + 255) // 256 * 256
 
 
+def merge4_samples(n, nv):
+    """Samples of the largest four-way pass: one per tile of X and of Y per
+    group of four runs (levels taken two at a time, after a two-way level
+    when their number is odd)."""
+    levels = max(0, (-(-n // nv) - 1).bit_length())
+    most = 0
+    for level in range(levels & 1, levels, 2):
+        run = nv << level
+        most = max(most, 2 * (-(-n // (4 * run))) * (2 * run // nv))
+    return most
+
+
 def test_sizing_mergesort_matches_formula():
     lib = mgpu_hip.load()
     n = 1 << 20
     nv = 512 * 19  # mergesort tile of 32-bit keys
     tiles = -(-n // nv)
+    assert merge4_samples(n, nv) == 128
+    parts = r256(16 * (tiles + 1)) + r256(8 * merge4_samples(n, nv))
     st, sz = _size(lib.mgpu_mergesort_kv_u32_u32, ctypes.c_void_p(1), ctypes.c_void_p(1), n, 0)
     assert st == 0
-    assert sz == r256(8 * (tiles + 1)) + r256(4 * n) + r256(4 * n)
+    assert sz == parts + r256(4 * n) + r256(4 * n)
     st, sz1 = _size(lib.mgpu_mergesort_keys_u32, ctypes.c_void_p(1), n, 0)
-    assert sz1 == r256(8 * (tiles + 1)) + r256(4 * n)
+    assert sz1 == parts + r256(4 * n)
     # One tile: no aux bytes (vk/mergesort.hxx:22-23).
     st, sz = _size(lib.mgpu_mergesort_keys_u32, ctypes.c_void_p(1), 100, 0)
     assert st == 0 and sz == 0

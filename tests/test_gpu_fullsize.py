@@ -14,7 +14,8 @@ N_SORT = 1 << 28
 
 
 @pytest.mark.parametrize("kind", ["u32", "lowentropy"])
-@pytest.mark.parametrize("algo", ["radix_count_scan_scatter", "radix_single_pass", "mergesort"])
+@pytest.mark.parametrize("algo", ["radix_count_scan_scatter", "radix_single_pass", "mergesort",
+                                  "mergesort_two_way"])
 def test_sort_2p28_kv(kind, algo):
     banner("fullsize sort", algo, kind)
     keys_in = gen_dev(N_SORT, kind, 0)
@@ -28,8 +29,15 @@ def test_sort_2p28_kv(kind, algo):
             mgpu_hip.radix_sort(keys, vals)
         finally:
             mgpu_hip.set_radix_algorithm(old)
+    elif algo == "mergesort_two_way":
+        old = mgpu_hip.mergesort_passes()
+        mgpu_hip.set_mergesort_passes(mgpu_hip.MERGESORT_TWO_WAY)
+        try:
+            mgpu_hip.mergesort_kv(keys, vals)
+        finally:
+            mgpu_hip.set_mergesort_passes(old)
     else:
-        mgpu_hip.mergesort_kv(keys, vals)
+        mgpu_hip.mergesort_kv(keys, vals)  # automatic: four-way passes at this size
     torch.cuda.synchronize()
     check_stable_sort_properties(keys_in, keys, vals)
 

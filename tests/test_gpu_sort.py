@@ -17,6 +17,9 @@ MERGE_NV = 512 * 23   # merge tile of 32-bit keys
 RADIX_NV = 512 * 8    # radix tile of every size below 2^22 keys
 SIZES = [1, 2, 7, 64, 100, MS_NV64 - 1, MS_NV64, MS_NV64 + 1, MS_NV - 1, MS_NV, MS_NV + 1,
          2 * MS_NV + 3, 5 * MS_NV - 11, RADIX_NV + 1, 100003, (1 << 20) + 17]
+# Tile counts whose merge levels end in partial groups of four runs (a last
+# group holding one, two or three runs, some of them short).
+GROUP_SIZES = [9 * MS_NV - 5, 17 * MS_NV + 2, 33 * MS_NV, 50 * MS_NV + 77, 130 * MS_NV - 1]
 
 
 def make_keys(kt, n, seed, pattern="random"):
@@ -50,12 +53,23 @@ def radix_algo(request):
     mgpu_hip.set_radix_algorithm(old)
 
 
+@pytest.fixture(params=[mgpu_hip.MERGESORT_TWO_WAY, mgpu_hip.MERGESORT_FOUR_WAY],
+                ids=["two_way", "four_way"])
+def merge_passes(request):
+    """Runs the test under each merge pass structure (at these sizes the
+    automatic choice would always be two-way)."""
+    old = mgpu_hip.mergesort_passes()
+    mgpu_hip.set_mergesort_passes(request.param)
+    yield request.param
+    mgpu_hip.set_mergesort_passes(old)
+
+
 # ------------------------------------------------------------------ mergesort
 
 @pytest.mark.parametrize("n", SIZES)
 @pytest.mark.parametrize("kt", KTS)
-def test_mergesort_kv(n, kt):
-    banner("mergesort_kv", kt, n)
+def test_mergesort_kv(n, kt, merge_passes):
+    banner("mergesort_kv", kt, n, merge_passes)
     keys = make_keys(kt, n, 1, "ties")
     vals = np.arange(n, dtype=np.uint32)
     dk, dv = dev_keys(keys), to_dev(vals)
@@ -65,11 +79,25 @@ def test_mergesort_kv(n, kt):
     np.testing.assert_array_equal(to_host(dv), ev)
 
 
+@pytest.mark.parametrize("n", GROUP_SIZES)
+@pytest.mark.parametrize("kt", ["u32", "f64"])
+@pytest.mark.parametrize("comp", ["less", "greater"])
+def test_mergesort_kv_partial_groups(n, kt, comp, merge_passes):
+    banner("mergesort_kv groups", kt, n, comp, merge_passes)
+    keys = make_keys(kt, n, 5, "ties" if comp == "less" else "random")
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = dev_keys(keys), to_dev(vals)
+    mgpu_hip.mergesort_kv(dk, dv, comp=comp)
+    ek, ev = oracle.stable_sort(keys, vals, kt, greater=(comp == "greater"))
+    np.testing.assert_array_equal(to_host(dk), ek)
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
 @pytest.mark.parametrize("pattern", ["random", "equal", "sorted", "reversed", "lowentropy"])
 @pytest.mark.parametrize("comp", ["less", "greater"])
-def test_mergesort_patterns(pattern, comp):
+def test_mergesort_patterns(pattern, comp, merge_passes):
     n = 3 * MS_NV + 1001
-    banner("mergesort pattern", pattern, comp)
+    banner("mergesort pattern", pattern, comp, merge_passes)
     keys = make_keys("u32", n, 2, pattern)
     vals = np.arange(n, dtype=np.uint32)
     dk, dv = dev_keys(keys), to_dev(vals)
@@ -81,9 +109,9 @@ def test_mergesort_patterns(pattern, comp):
 
 @pytest.mark.parametrize("kt", KTS)
 @pytest.mark.parametrize("comp", ["less", "greater"])
-def test_mergesort_keys(kt, comp):
+def test_mergesort_keys(kt, comp, merge_passes):
     n = 200001
-    banner("mergesort_keys", kt, comp)
+    banner("mergesort_keys", kt, comp, merge_passes)
     keys = make_keys(kt, n, 3)
     dk = dev_keys(keys)
     mgpu_hip.mergesort_keys(dk, comp=comp)
