@@ -5,13 +5,16 @@
 //      network over vt registers (common/sort_networks.hxx:11-39, with the
 //      partial-tile mask of common/cta_mergesort.hxx:7-15) and log2(nt) merge
 //      passes in LDS (common/cta_mergesort.hxx:87-133);
-//   2. ceil(log2(tiles)) global passes; each pass partitions the coop groups of
-//      2<<pass tiles with merge path (vk/mergesort.hxx:65-72) and merges one
-//      output tile per block (common/kernel_mergesort.hxx:72-121).
+//   2. ceil(log2(tiles)) levels of merging.  The reference runs one global
+//      pass per level: partition the coop groups of 2<<pass tiles with merge
+//      path (vk/mergesort.hxx:65-72), then merge one output tile per block
+//      (common/kernel_mergesort.hxx:72-121).  Large sorts here merge two
+//      levels per pass (four runs into one, below), so 2^28 pairs take 8
+//      global passes instead of 15; small ones keep one level per pass.
 // The ping-pong parity lands the result in the caller's buffers
 // (vk/mergesort.hxx:47-48).  Every comparison is strict (swap only when
-// comp(b, a)), and merges take A on ties, so the sort is stable and its output
-// equals std::stable_sort.
+// comp(b, a)), and merges take the earlier input on ties, so the sort is
+// stable and its output equals std::stable_sort.
 #pragma once
 #include "merge.hxx"
 
