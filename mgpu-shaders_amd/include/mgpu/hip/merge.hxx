@@ -3,9 +3,12 @@
 // merge_path follows inc/mgpu/common/cta_merge.hxx:10-31 with bounds_lower
 // (ties go to A): it returns the number of A elements among the first `diag`
 // outputs of the stable merge.  The partition search is one binary search
-// per thread in global memory, as in vk/partition.hxx:9-20 (a wave-wide
-// 64-probe search was measured: fewer dependent rounds, but four times the
-// memory traffic, and slower).
+// per thread in global memory, as in vk/partition.hxx:9-20.  Searches with
+// more probes per round take fewer dependent rounds but fetch more scattered
+// cache lines, and on MI355X the line fetches cost more than the rounds save:
+// a wave-wide 64-probe search was four times the traffic and slower, and 3 or
+// 7 probes per round took the merge partition of 2 x 2^27 keys from 0.043 to
+// 0.061 or 0.093 ms.
 //
 // A tile merge (kernel_merge.hxx:9-58 and cta_merge.hxx:136-162) stages the A
 // and B slices of one output tile in LDS with coalesced loads, runs one merge
