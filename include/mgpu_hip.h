@@ -43,6 +43,13 @@ extern "C" {
 /* hipStream_t without pulling in the HIP headers (same opaque type). */
 typedef struct ihipStream_t* mgpu_stream_t;
 
+/* Every enum below spans the whole int32 range (the *_ENUM_MIN / *_ENUM_MAX
+ * enumerators name no value): any int a C or FFI caller passes is then a
+ * valid value of the type, and one outside the named values reaches the
+ * library's argument check (MGPU_INVALID_ARG) instead of being undefined in
+ * the C++ that implements the entry points.                                */
+#define MGPU_ENUM_RANGE(prefix) prefix##_ENUM_MIN = -0x7fffffff - 1, prefix##_ENUM_MAX = 0x7fffffff
+
 typedef enum mgpu_status_t {
   MGPU_OK = 0,
   MGPU_INVALID_ARG = 1,   /* negative count, bad enum, aux not 256-byte
@@ -51,15 +58,18 @@ typedef enum mgpu_status_t {
                              aux == NULL never reads data pointers)          */
   MGPU_AUX_TOO_SMALL = 2, /* *aux_size (capacity) below the requirement          */
   MGPU_HIP_ERROR = 3,     /* a HIP runtime call failed; see mgpu_last_hip_error  */
-  MGPU_UNSUPPORTED = 4    /* size beyond a documented limit of the entry point   */
+  MGPU_UNSUPPORTED = 4,   /* size beyond a documented limit of the entry point   */
+  MGPU_ENUM_RANGE(MGPU_STATUS)
 } mgpu_status_t;
 
 /* Comparator for mergesort / merge (inc/mgpu/vk/mergesort.hxx:11 takes any
  * functor; the C ABI exports the two standard ones, custom functors use the
  * C++ templates in mgpu/hip/ headers). */
-typedef enum mgpu_comp_t { MGPU_LESS = 0, MGPU_GREATER = 1 } mgpu_comp_t;
+typedef enum mgpu_comp_t { MGPU_LESS = 0, MGPU_GREATER = 1, MGPU_ENUM_RANGE(MGPU_COMP) } mgpu_comp_t;
 
-typedef enum mgpu_scan_type_t { MGPU_SCAN_EXCLUSIVE = 0, MGPU_SCAN_INCLUSIVE = 1 } mgpu_scan_type_t;
+typedef enum mgpu_scan_type_t {
+  MGPU_SCAN_EXCLUSIVE = 0, MGPU_SCAN_INCLUSIVE = 1, MGPU_ENUM_RANGE(MGPU_SCAN_TYPE)
+} mgpu_scan_type_t;
 
 const char* mgpu_status_string(mgpu_status_t status);
 /* hipError_t of the last MGPU_HIP_ERROR raised on the calling thread. */
@@ -139,7 +149,9 @@ MGPU_DECL_MERGE(f64, double)
  * num_merge_partitions (cta_merge.hxx:6-8) entries of int64.  LOWER: ties
  * take A first (the stable merge); UPPER: ties take B first.  No aux; the
  * caller sizes mp.  spacing > 0.                                          */
-typedef enum mgpu_bounds_t { MGPU_BOUNDS_LOWER = 0, MGPU_BOUNDS_UPPER = 1 } mgpu_bounds_t;
+typedef enum mgpu_bounds_t {
+  MGPU_BOUNDS_LOWER = 0, MGPU_BOUNDS_UPPER = 1, MGPU_ENUM_RANGE(MGPU_BOUNDS)
+} mgpu_bounds_t;
 #define MGPU_DECL_PARTITIONS(K, KT)                                            \
   mgpu_status_t mgpu_merge_path_partitions_##K(mgpu_stream_t stream,         \
       const KT* a, int64_t a_count, const KT* b, int64_t b_count, int64_t* mp, \
@@ -300,9 +312,12 @@ void mgpu_graph_destroy(mgpu_graph_t graph);
  *          f32: inclusive of U[0, 1) (multiples of 2^-24)
  * Timestamps are host steady-clock milliseconds since creation.            */
 typedef struct mgpu_server_st* mgpu_server_t;
-typedef enum mgpu_req_op_t { MGPU_REQ_SORT = 0, MGPU_REQ_MERGE = 1, MGPU_REQ_SCAN = 2 } mgpu_req_op_t;
+typedef enum mgpu_req_op_t {
+  MGPU_REQ_SORT = 0, MGPU_REQ_MERGE = 1, MGPU_REQ_SCAN = 2, MGPU_ENUM_RANGE(MGPU_REQ_OP)
+} mgpu_req_op_t;
 typedef enum mgpu_key_type_t {
-  MGPU_KEY_U32 = 0, MGPU_KEY_I32 = 1, MGPU_KEY_U64 = 2, MGPU_KEY_F32 = 3
+  MGPU_KEY_U32 = 0, MGPU_KEY_I32 = 1, MGPU_KEY_U64 = 2, MGPU_KEY_F32 = 3,
+  MGPU_ENUM_RANGE(MGPU_KEY_TYPE)
 } mgpu_key_type_t;
 typedef struct mgpu_request_t {
   int32_t op;        /* mgpu_req_op_t   */
@@ -386,7 +401,8 @@ typedef enum mgpu_gen_kind_t {
   MGPU_GEN_IOTA = 6,       /* element i = i                                  */
   MGPU_GEN_SORTED_U32 = 7, /* ascending: i*s + rand % s, s = 2^32 / count    */
   MGPU_GEN_SORTED_U64 = 8, /* ascending: i*s + rand % s, s = 2^64 / count    */
-  MGPU_GEN_SORTED_F32 = 9  /* ascending: (i + U[0,1)) / count                */
+  MGPU_GEN_SORTED_F32 = 9, /* ascending: (i + U[0,1)) / count                */
+  MGPU_ENUM_RANGE(MGPU_GEN)
 } mgpu_gen_kind_t;
 mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
                             mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset);

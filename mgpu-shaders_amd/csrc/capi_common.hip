@@ -20,6 +20,7 @@ const char* mgpu_status_string(mgpu_status_t status) {
     case MGPU_AUX_TOO_SMALL: return "MGPU_AUX_TOO_SMALL";
     case MGPU_HIP_ERROR: return "MGPU_HIP_ERROR";
     case MGPU_UNSUPPORTED: return "MGPU_UNSUPPORTED";
+    default: break;
   }
   return "MGPU_UNKNOWN_STATUS";
 }
