@@ -14,9 +14,14 @@ The JSON line also carries:
                 value) / its average duration from the library's per-launch
                 HIP events, against the 8 TB/s HBM peak; `traffic` is read
                 from profiles/traffic.json when a rocprofv3 PMC measurement
-                of the same kernel has been recorded there;
+                of the same kernel has been recorded there; `copy_ceiling`
+                is a plain 2 GiB device-to-device copy timed the same way
+                (context for the memory-bound kernels, not the denominator);
   cpu_baseline  std::stable_sort of (key, index) pairs from the CPU oracle,
                 one core, on a bounded sample of the same distribution;
+  cpu_baseline_parallel  the same pairs sorted by up to 16 std::threads
+                (chunk std::stable_sort + merge-path-split std::merge rounds),
+                labelled separately;
   segsort       65536 lognormal segments (~2^25.6 keys): batched segmented
                 sort (eager and graph replay) vs a loop of plain sorts;
   secondary     the other primitives at their workload sizes (mergesort kv
@@ -114,6 +119,57 @@ def cpu_baseline(log2n, reps=3):
                       "(seed 0), median of %d runs (%s s) on one core, nproc=%d"
                       % (log2n, reps, ", ".join("%.2f" % t for t in ts), os.cpu_count() or 0),
             "seconds": ts, "cpu_model": cpu_model()}
+
+
+def cpu_baseline_parallel(log2n, reps=3, threads=None):
+    """A separately labelled multi-threaded CPU baseline (the oracle): the
+    same 2^log2n (key, index) pairs stable-sorted by `threads` std::threads,
+    one std::stable_sort per chunk, then merge rounds each split across the
+    threads by merge path; median of `reps` runs."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    import oracle
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    n = 1 << log2n
+    keys = oracle.generate(n, "u32", 0)
+    vals = np.arange(n, dtype=np.uint32)
+    ts = [oracle.stable_sort_threads(keys, vals, "u32", False, threads)[1] for _ in range(reps)]
+    dt = sorted(ts)[len(ts) // 2]
+    return {"value": n / dt, "unit": "keys/s", "cores": threads, "kind": "port",
+            "sample": "2^%d (u32 key, u32 index) pairs, uniform keys (seed 0), %d std::threads: "
+                      "std::stable_sort per chunk, then std::merge rounds split by merge path; "
+                      "median of %d runs (%s s), nproc=%d"
+                      % (log2n, threads, reps, ", ".join("%.2f" % t for t in ts),
+                         os.cpu_count() or 0),
+            "seconds": ts}
+
+
+def copy_ceiling(torch, steps=10):
+    """A plain device-to-device copy of 2 GiB (torch copy_, a HIP blit kernel)
+    timed with events: the practical read + write ceiling the memory-bound
+    kernels are compared with (fractions stay against the 8 TB/s peak)."""
+    n = 1 << 29
+    src = torch.empty(n, dtype=torch.uint32, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = sorted(ms)[len(ms) // 2]
+    moved = 2 * 4 * n
+    del src, dst
+    return {"bytes_moved": moved, "ms": t, "GBps": moved / (t / 1e3) / 1e9,
+            "frac_of_peak": moved / (t / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "what": "device-to-device copy of 2^29 uint32 (2 GiB read + 2 GiB written), median "
+                    "of %d" % steps}
 
 
 def cpu_model():
@@ -827,6 +883,11 @@ def main():
     }
     del keys_src, vals_src, keys, vals, aux
     torch.cuda.empty_cache()
+    cc = copy_ceiling(torch)
+    cc["scatter_frac_of_copy"] = achieved / cc["GBps"]
+    result["roofline"]["copy_ceiling"] = cc
+    log("copy ceiling", cc)
+    torch.cuda.empty_cache()
     if not args.no_secondary:
         if not args.no_segsort:
             result["segsort"] = bench_segsort(torch, mgpu_hip, max(3, min(args.steps, 10)), 2,
@@ -848,6 +909,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline 2^%d x %d" % (args.cpu_log2, args.cpu_reps))
         result["cpu_baseline"] = cpu_baseline(args.cpu_log2, args.cpu_reps)
+        log("cpu baseline, threads")
+        result["cpu_baseline_parallel"] = cpu_baseline_parallel(args.cpu_log2, args.cpu_reps)
         if not args.no_secondary:
             result["cpu_baselines_other"] = cpu_baselines_other()
     print(json.dumps(result), flush=True)

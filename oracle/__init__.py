@@ -54,6 +54,8 @@ def lib():
         L.oracle_stable_sort.argtypes = [c_int, vp, vp, i64, c_int]
         L.oracle_stable_sort_timed.argtypes = [c_int, vp, vp, i64, c_int]
         L.oracle_stable_sort_timed.restype = ctypes.c_double
+        L.oracle_stable_sort_threads_timed.argtypes = [c_int, vp, vp, i64, c_int, c_int]
+        L.oracle_stable_sort_threads_timed.restype = ctypes.c_double
         L.oracle_stable_sort_radix.argtypes = [c_int, vp, vp, i64]
         L.oracle_segmented_stable_sort.argtypes = [c_int, vp, vp, i64, vp, i64, c_int]
         L.oracle_std_sort_u32.argtypes = [vp, i64]
@@ -163,6 +165,18 @@ def stable_sort_timed(keys, vals=None, kt="u32", greater=False):
     if v is not None and len(v) != len(k):
         raise ValueError("one value per key")
     return float(lib().oracle_stable_sort_timed(KT[kt], _p(k), _p(v), len(k), int(greater)))
+
+
+def stable_sort_threads(keys, vals=None, kt="u32", greater=False, threads=4):
+    """The multi-threaded stable sort (one std::stable_sort per chunk, then
+    merge rounds split by merge path): ((keys, vals) sorted, seconds)."""
+    k = np.ascontiguousarray(keys, dtype=NP[kt]).copy()
+    v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    if v is not None and len(v) != len(k):
+        raise ValueError("one value per key")
+    dt = float(lib().oracle_stable_sort_threads_timed(KT[kt], _p(k), _p(v), len(k), int(greater),
+                                                       int(threads)))
+    return ((k, v) if vals is not None else k), dt
 
 
 def stable_sort_radix(keys, vals=None, kt="u32"):

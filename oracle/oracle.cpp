@@ -6,6 +6,9 @@
 //
 // Two kinds of functions:
 //   * standard oracles: std::stable_sort, std::merge, std::exclusive_scan /
+//     (and a multi-threaded stable sort built from them, a separately
+//     labelled CPU baseline: std::stable_sort of one chunk per thread, then
+//     std::merge rounds, each merge split across the threads by merge path),
 //     std::inclusive_scan in uint32 modular arithmetic, float64 prefix sums,
 //     std::reduce, and their per-segment forms for the segmented sort and
 //     scan (the reference's own CPU comparators: std::sort in
@@ -38,6 +41,7 @@
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -123,6 +127,74 @@ double stable_sort_kv(K* keys, uint32_t* vals, int64_t n, int greater) {
   for (int64_t i = 0; i < n; ++i) {
     keys[i] = v[i].first;
     if (vals) vals[i] = v[i].second;
+  }
+  return dt;
+}
+
+// Stable merge of sorted a and b into out (ties take a), split across
+// `threads` std::threads at merge-path diagonals: each thread std::merge()s
+// the slices between its two diagonals.
+template <typename P, typename Cmp>
+void parallel_merge(const P* a, int64_t na, const P* b, int64_t nb, P* out, int threads, Cmp cmp) {
+  auto split = [&](int64_t d) {
+    int64_t lo = std::max<int64_t>(0, d - nb), hi = std::min(d, na);
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (!cmp(b[d - 1 - mid], a[mid])) lo = mid + 1;
+      else hi = mid;
+    }
+    return lo;
+  };
+  std::vector<std::thread> ts;
+  for (int t = 0; t < threads; ++t) {
+    const int64_t d0 = (na + nb) * t / threads, d1 = (na + nb) * (t + 1) / threads;
+    const int64_t a0 = split(d0), a1 = split(d1);
+    ts.emplace_back([=] { std::merge(a + a0, a + a1, b + (d0 - a0), b + (d1 - a1), out + d0, cmp); });
+  }
+  for (auto& t : ts) t.join();
+}
+
+// Multi-threaded stable sort of (key, value) pairs: one chunk per thread
+// sorted by std::stable_sort, then rounds of parallel_merge of adjacent runs
+// (the earlier run first, so ties keep their order).  Returns the seconds
+// from the first chunk sort to the last merge.
+template <typename K>
+double stable_sort_kv_threads(K* keys, uint32_t* vals, int64_t n, int greater, int threads) {
+  typedef std::pair<K, uint32_t> P;
+  std::vector<P> v((size_t)n), w((size_t)n);
+  for (int64_t i = 0; i < n; ++i) v[i] = {keys[i], vals ? vals[i] : 0u};
+  comp_t<K> c{greater};
+  auto cmp = [&](const P& a, const P& b) { return c(a.first, b.first); };
+  threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n));
+  std::vector<int64_t> bounds;
+  for (int t = 0; t <= threads; ++t) bounds.push_back(n * t / threads);
+  const auto t0 = std::chrono::steady_clock::now();
+  {
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+      ts.emplace_back([&, t] { std::stable_sort(v.begin() + bounds[t], v.begin() + bounds[t + 1], cmp); });
+    for (auto& t : ts) t.join();
+  }
+  P* src = v.data();
+  P* dst = w.data();
+  while (bounds.size() > 2) {
+    std::vector<int64_t> next;
+    for (size_t r = 0; r + 1 < bounds.size(); r += 2) {
+      next.push_back(bounds[r]);
+      if (r + 2 < bounds.size())
+        parallel_merge(src + bounds[r], bounds[r + 1] - bounds[r], src + bounds[r + 1],
+                       bounds[r + 2] - bounds[r + 1], dst + bounds[r], threads, cmp);
+      else
+        std::copy(src + bounds[r], src + bounds[r + 1], dst + bounds[r]);
+    }
+    next.push_back(n);
+    bounds.swap(next);
+    std::swap(src, dst);
+  }
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int64_t i = 0; i < n; ++i) {
+    keys[i] = src[i].first;
+    if (vals) vals[i] = src[i].second;
   }
   return dt;
 }
@@ -464,6 +536,17 @@ double oracle_stable_sort_timed(int kt, void* keys, uint32_t* vals, int64_t n, i
   dispatch_key(kt, [&](auto* tag) {
     typedef std::remove_pointer_t<decltype(tag)> K;
     dt = stable_sort_kv((K*)keys, vals, n, greater);
+  });
+  return dt;
+}
+
+// The multi-threaded stable sort (same result); returns its seconds.
+double oracle_stable_sort_threads_timed(int kt, void* keys, uint32_t* vals, int64_t n,
+                                        int greater, int threads) {
+  double dt = 0;
+  dispatch_key(kt, [&](auto* tag) {
+    typedef std::remove_pointer_t<decltype(tag)> K;
+    dt = stable_sort_kv_threads((K*)keys, vals, n, greater, threads);
   });
   return dt;
 }

@@ -16,6 +16,8 @@ void oracle_generate(void* out, int64_t count, int elem_bytes, int kind, uint64_
 void oracle_glibc_rand(int64_t n, uint32_t* out);
 void oracle_stable_sort(int kt, void* keys, uint32_t* vals, int64_t n, int greater);
 double oracle_stable_sort_timed(int kt, void* keys, uint32_t* vals, int64_t n, int greater);
+double oracle_stable_sort_threads_timed(int kt, void* keys, uint32_t* vals, int64_t n, int greater,
+                                        int threads);
 void oracle_segmented_stable_sort(int kt, void* keys, uint32_t* vals, int64_t n,
                                   const int32_t* segs, int64_t num_segments, int greater);
 void oracle_stable_sort_radix(int kt, void* keys, uint32_t* vals, int64_t n);
@@ -97,6 +99,13 @@ int main() {
         (void)oracle_stable_sort_timed(kt, k3.data(), v3.data(), n, greater);
         EXPECT(std::memcmp(k1.data(), k3.data(), (size_t)(n * eb)) == 0 && v1 == v3,
                "timed sort kt=%d n=%lld", kt, (long long)n);
+        for (int threads : {1, 3, 8}) {
+          std::vector<uint8_t> k8 = keys;
+          std::vector<uint32_t> v8 = vals;
+          (void)oracle_stable_sort_threads_timed(kt, k8.data(), v8.data(), n, greater, threads);
+          EXPECT(std::memcmp(k1.data(), k8.data(), (size_t)(n * eb)) == 0 && v1 == v8,
+                 "threaded sort kt=%d n=%lld threads=%d", kt, (long long)n, threads);
+        }
         // Merge of the two sorted halves, and merge path at every diagonal
         // for both bounds.
         const int64_t na = n / 3, nb = n - na;

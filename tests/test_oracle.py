@@ -525,3 +525,24 @@ def test_oracle_under_address_and_undefined_behavior_sanitizers():
                        timeout=600)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "oracle sanitize: ALL PASS" in r.stdout
+
+
+@pytest.mark.parametrize("kt", ["u32", "f64"])
+@pytest.mark.parametrize("greater", [False, True])
+@pytest.mark.parametrize("n,threads", [(0, 4), (1, 4), (7, 3), (5000, 8), (100003, 5),
+                                        (65536, 16)])
+def test_stable_sort_threads_equals_stable_sort(kt, greater, n, threads):
+    """The multi-threaded CPU baseline (chunk sorts + merge-path-split merge
+    rounds) gives the stable sort: checked against numpy's stable argsort on
+    tie-heavy keys, and against the single-threaded oracle."""
+    keys = keys_for(kt, n, 9, low_entropy=True)
+    keys = keys[np.arange(n) % 61] if n else keys
+    vals = np.arange(n, dtype=np.uint32)
+    (k, v), dt = oracle.stable_sort_threads(keys, vals, kt, greater, threads)
+    assert dt >= 0.0
+    order = np_stable_order(keys, kt, greater)
+    np.testing.assert_array_equal(v, vals[order])
+    np.testing.assert_array_equal(k, keys[order])
+    ek, ev = oracle.stable_sort(keys, vals, kt, greater)
+    np.testing.assert_array_equal(k, ek)
+    np.testing.assert_array_equal(v, ev)
