@@ -15,8 +15,9 @@ The JSON line also carries:
                 HIP events, against the 8 TB/s HBM peak; `traffic` is read
                 from profiles/traffic.json when a rocprofv3 PMC measurement
                 of the same kernel has been recorded there; `copy_ceiling`
-                is a plain 2 GiB device-to-device copy timed the same way
-                (context for the memory-bound kernels, not the denominator);
+                is a plain 2 GiB device-to-device copy by the library's copy
+                kernel (and by the HIP runtime) timed the same way: context
+                for the memory-bound kernels, not the denominator;
   cpu_baseline  std::stable_sort of (key, index) pairs from the CPU oracle,
                 one core, on a bounded sample of the same distribution;
   cpu_baseline_parallel  the same pairs sorted by up to 16 std::threads
@@ -144,32 +145,41 @@ def cpu_baseline_parallel(log2n, reps=3, threads=None):
             "seconds": ts}
 
 
-def copy_ceiling(torch, steps=10):
-    """A plain device-to-device copy of 2 GiB (torch copy_, a HIP blit kernel)
-    timed with events: the practical read + write ceiling the memory-bound
-    kernels are compared with (fractions stay against the 8 TB/s peak)."""
+def copy_ceiling(torch, mgpu_hip, steps=10):
+    """Plain device-to-device copies of 2 GiB timed with events: the library's
+    HBM copy kernel (16 KB tiles, non-temporal loads; the practical read +
+    write ceiling the memory-bound kernels are compared with) and the HIP
+    runtime's copy (torch copy_) beside it.  Fractions stay against the
+    8 TB/s peak."""
     n = 1 << 29
     src = torch.empty(n, dtype=torch.uint32, device="cuda")
     dst = torch.empty_like(src)
     src.fill_(1)
-    for _ in range(3):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    ms = []
-    for _ in range(steps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        dst.copy_(src)
-        e1.record()
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
         torch.cuda.synchronize()
-        ms.append(e0.elapsed_time(e1))
-    t = sorted(ms)[len(ms) // 2]
+        ms = []
+        for _ in range(steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return sorted(ms)[len(ms) // 2]
+
     moved = 2 * 4 * n
+    t = timed(lambda: mgpu_hip.copy(dst, src))
+    ok = bool(torch.equal(dst, src))
+    t_rt = timed(lambda: dst.copy_(src))
     del src, dst
     return {"bytes_moved": moved, "ms": t, "GBps": moved / (t / 1e3) / 1e9,
-            "frac_of_peak": moved / (t / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "what": "device-to-device copy of 2^29 uint32 (2 GiB read + 2 GiB written), median "
-                    "of %d" % steps}
+            "frac_of_peak": moved / (t / 1e3) / 1e9 / HBM_PEAK_GBS, "copy_correct": ok,
+            "runtime_copy_ms": t_rt, "runtime_copy_GBps": moved / (t_rt / 1e3) / 1e9,
+            "what": "copy of 2^29 uint32 (2 GiB read + 2 GiB written) by mgpu_copy, and by the "
+                    "HIP runtime (torch copy_) for comparison; medians of %d" % steps}
 
 
 def cpu_model():
@@ -883,7 +893,7 @@ def main():
     }
     del keys_src, vals_src, keys, vals, aux
     torch.cuda.empty_cache()
-    cc = copy_ceiling(torch)
+    cc = copy_ceiling(torch, mgpu_hip)
     cc["scatter_frac_of_copy"] = achieved / cc["GBps"]
     result["roofline"]["copy_ceiling"] = cc
     log("copy ceiling", cc)

@@ -406,6 +406,11 @@ typedef enum mgpu_gen_kind_t {
 } mgpu_gen_kind_t;
 mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int elem_bytes,
                             mgpu_gen_kind_t kind, uint64_t seed, uint64_t offset);
+/* Device-to-device copy of `bytes` (dst and src 16-byte aligned, not
+ * overlapping): a plain HBM copy kernel, the practical read + write ceiling
+ * that the memory-bound primitives are compared with.  No counterpart in the
+ * reference.                                                               */
+mgpu_status_t mgpu_copy(mgpu_stream_t stream, void* dst, const void* src, int64_t bytes);
 
 /* ------------------------------------------------------------------------ */
 /* Look-back control (no counterpart in the reference, which has no single-

@@ -4,6 +4,7 @@
 #include "mgpu/hip/generate.hxx"
 #include "mgpu/hip/meta.hxx"
 #include "mgpu/hip/segsort.hxx"
+#include "mgpu/hip/transform.hxx"
 
 using namespace mgpu::hip;
 
@@ -83,6 +84,10 @@ mgpu_status_t mgpu_generate(mgpu_stream_t stream, void* out, int64_t count, int 
   if (count == 0) return MGPU_OK;
   if (!out) return MGPU_INVALID_ARG;
   return generate(out, count, elem_bytes, (int)kind, seed, offset, (hipStream_t)stream);
+}
+
+mgpu_status_t mgpu_copy(mgpu_stream_t stream, void* dst, const void* src, int64_t bytes) {
+  return copy_bytes(dst, src, bytes, (hipStream_t)stream);
 }
 
 void mgpu_set_lookback_spin_limit(uint32_t limit) {

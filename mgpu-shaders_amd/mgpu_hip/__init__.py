@@ -117,6 +117,7 @@ def load():
     lib.mgpu_graph_destroy.argtypes = [vp]
     lib.mgpu_generate.restype = c_int
     lib.mgpu_generate.argtypes = [vp, vp, i64, c_int, c_int, ctypes.c_uint64, ctypes.c_uint64]
+    lib.mgpu_copy.argtypes = [vp, vp, vp, i64]
     lib.mgpu_set_radix_algorithm.restype = None
     lib.mgpu_set_radix_algorithm.argtypes = [c_int]
     lib.mgpu_set_mergesort_passes.restype = None
@@ -511,6 +512,18 @@ def generate(out, kind, seed=0, offset=0, stream=None):
     _check(load().mgpu_generate(_stream(stream), _ptr(out), out.numel(), out.element_size(),
                                 GEN_KINDS[kind], seed, offset), "generate")
     return out
+
+
+def copy(dst, src, stream=None):
+    """dst := src (contiguous device tensors of the same byte size, 16-byte
+    aligned) with the library's plain HBM copy kernel."""
+    _device_check(dst, src)
+    nbytes = src.numel() * src.element_size()
+    if not (dst.is_contiguous() and src.is_contiguous()) or \
+            dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("copy needs contiguous tensors of the same byte size")
+    _check(load().mgpu_copy(_stream(stream), _ptr(dst), _ptr(src), nbytes), "copy")
+    return dst
 
 
 RADIX_AUTO, RADIX_COUNT_SCAN_SCATTER, RADIX_SINGLE_PASS = 0, 1, 2

@@ -89,3 +89,4 @@ def test_caller_aux_used_on_side_stream_is_recorded():
     torch.cuda.synchronize()
     assert np.array_equal(dk.cpu().numpy(), ek) and np.array_equal(dv.cpu().numpy(), ev)
     del junk
+
