@@ -13,8 +13,17 @@ from gpu_util import banner, to_dev, to_host
 pytestmark = pytest.mark.gpu
 
 
-def test_primitives_graph_replay():
-    banner("primitives graph replay")
+@pytest.fixture(params=[mgpu_hip.MERGESORT_TWO_WAY, mgpu_hip.MERGESORT_FOUR_WAY],
+                ids=["two_way", "four_way"])
+def merge_passes(request):
+    old = mgpu_hip.mergesort_passes()
+    mgpu_hip.set_mergesort_passes(request.param)
+    yield request.param
+    mgpu_hip.set_mergesort_passes(old)
+
+
+def test_primitives_graph_replay(merge_passes):
+    banner("primitives graph replay", merge_passes)
     n = (1 << 20) + 3       # several tiles of every kernel, a partial last tile
     na, nb = 300007, 200003
     stream = torch.cuda.Stream()

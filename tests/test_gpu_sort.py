@@ -118,6 +118,23 @@ def test_mergesort_keys(kt, comp, merge_passes):
     np.testing.assert_array_equal(to_host(dk), oracle.stable_sort(keys, kt=kt, greater=(comp == "greater")))
 
 
+@pytest.mark.parametrize("kt", ["f32", "u64"])
+@pytest.mark.parametrize("comp", ["less", "greater"])
+def test_mergesort_indices(kt, comp, merge_passes):
+    """indices[i] = i before the sort, then sorted with the keys: the stable
+    argsort, over enough tiles for two four-way passes."""
+    n = 100003
+    banner("mergesort_indices", kt, comp, merge_passes)
+    keys = make_keys(kt, n, 8, "ties")
+    dk = dev_keys(keys)
+    di = torch.empty(n, dtype=torch.int32, device="cuda")
+    mgpu_hip.mergesort_indices(dk, di, comp=comp)
+    vals = np.arange(n, dtype=np.uint32)
+    ek, ev = oracle.stable_sort(keys, vals, kt, greater=(comp == "greater"))
+    np.testing.assert_array_equal(to_host(dk), ek)
+    np.testing.assert_array_equal(to_host(di).view(np.uint32), ev)
+
+
 def test_mergesort_indices_gl_sort_known_answer():
     # examples/gl_sort/gl_sort.cxx:33-42: 10000 floats rand() % 10000 sorted
     # with indices; gathered == sorted and the output is sorted.
