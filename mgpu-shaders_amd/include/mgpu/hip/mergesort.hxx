@@ -275,7 +275,9 @@ constexpr int merge4_sample_div = 1;
 // Four-way passes pay off once a pass moves enough bytes to outweigh the
 // extra partition kernel: per-pass keys + values >= 256 MB (2^24 u32 pairs:
 // 0.81 ms two-way vs 0.92 four-way; 5 x 10^7 pairs 2.51 vs 1.93; 2^28 pairs
-// 13.5 vs 10.1).
+// 13.5 vs 10.1).  The same byte count marks the crossover for 64-bit keys
+// (2^24 u64 + u32 pairs, 201 MB: 1.36 vs 1.44 ms; 2^25 u64 keys, 268 MB:
+// 1.82 vs 1.84; 2^25 pairs, 402 MB: 2.54 vs 2.30; 2^27 pairs 9.75 vs 7.04).
 constexpr int64_t merge4_min_bytes = (int64_t)1 << 28;
 
 // Pass structure of the merge sort (a process-wide setting): 0 = automatic
