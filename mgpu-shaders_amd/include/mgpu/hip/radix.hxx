@@ -40,7 +40,7 @@ constexpr int num_xcds = 8;
 
 // Tile processed by block b when blocks are dealt round-robin to the XCDs:
 // XCD x = b % 8 gets the contiguous range [start(x), start(x) + share(x)).
-__device__ inline int64_t xcd_tile(int64_t b, int64_t num_tiles) {
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t num_tiles) {
   const int64_t q = num_tiles / num_xcds, r = num_tiles % num_xcds;
   const int64_t x = b % num_xcds, k = b / num_xcds;
   return x * q + min_(x, r) + k;
@@ -61,7 +61,7 @@ struct radix_bits_t {
 // keep same-address LDS atomics at most 4 deep even when every lane holds the
 // same digit.
 template <int CNT, int NV, typename key_t, typename xbits_t>
-__device__ inline void cta_radix_count(const key_t* keys, int tile_count, int shift, xbits_t xb,
+__device__ __forceinline__ void cta_radix_count(const key_t* keys, int tile_count, int shift, xbits_t xb,
                                        uint32_t* out, int64_t stride, uint32_t* h) {
   constexpr int COLS = 16;
   constexpr int KPV = 16 / (int)sizeof(key_t);   // keys per 128-bit vector
@@ -179,13 +179,16 @@ struct radix_fixed_offsets_t {
   __device__ uint32_t resolve(int, uint32_t) const { return goff; }
 };
 
-template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
-          typename xbits_t, typename offs_t>
-__device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
-                                         key_t* keys_out, uint32_t* vals_out, int tile_count,
-                                         int shift, xbits_t xb, offs_t& offs, int64_t out_limit,
-                                         radix_scatter_storage_t<NT, VT, key_t>& s,
-                                         uint8_t* digits_out = nullptr, int next_shift = 0) {
+// Loads and ranks one tile: k[i] gets the keys, r[i] = digit << 16 | the
+// key's rank among the same-digit keys of its wave, and s.hist[w][d] the
+// tile position of wave w's first key of digit d.  Thread d < 256 gets the
+// tile's count of digit d (total) and the tile position of its first key
+// (off_local).  Ends with a block barrier.
+template <int NT, int VT, typename key_t, bool full, typename xbits_t, typename offs_t>
+__device__ __forceinline__ void cta_radix_rank(const key_t* keys_in, int tile_count, int shift, xbits_t xb,
+                                      offs_t& offs, radix_scatter_storage_t<NT, VT, key_t>& s,
+                                      key_t (&k)[VT], uint32_t (&r)[VT], uint32_t& total,
+                                      uint32_t& off_local) {
   constexpr int NW = NT / warp_size;
   static_assert(NT >= radix_num_bins && NT % radix_num_bins == 0, "NT must cover the bins");
   static_assert(sizeof(key_t) * NT * VT >= sizeof(uint64_t) * NW * radix_num_bins,
@@ -201,7 +204,6 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   // [chunk + 64 i, chunk + 64 i + 64) in lane order, so wave-local rank order
   // (slot, lane) is the input order.
   const uint32_t chunk = (uint32_t)(warp * warp_size * VT) + (uint32_t)lane;
-  key_t k[VT];
 #pragma unroll
   for (int i = 0; i < VT; ++i)
     k[i] = full ? (keys_in + chunk)[i * warp_size]  // constant offsets from one address
@@ -232,7 +234,6 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   //    through LDS, so LDS is the tighter resource.
   // (Each was measured faster than the other for its key size on MI355X.)
   const uint64_t lane_bit = 1ull << lane;
-  uint32_t r[VT];
 #pragma unroll
   for (int i = 0; i < VT; ++i) {
     const uint32_t d = digit(i);
@@ -271,7 +272,7 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
   __syncthreads();
 
   // Per digit: exclusive prefix over waves, tile total, tile-local offsets.
-  uint32_t total = 0;
+  total = 0;
   if (tid < radix_num_bins) {
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
@@ -287,7 +288,7 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
     if (lane == warp_size - 1) s.scan_tot[warp] = incl;
   }
   __syncthreads();
-  uint32_t off_local = 0;
+  off_local = 0;
   if (tid < radix_num_bins) {
     off_local = incl - total;
     for (int w = 0; w < warp; ++w) off_local += s.scan_tot[w];
@@ -296,6 +297,25 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
     for (int w = 0; w < NW; ++w) s.hist[w][tid] += off_local;
   }
   __syncthreads();
+}
+
+template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
+          typename xbits_t, typename offs_t>
+__device__ __forceinline__ void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
+                                         key_t* keys_out, uint32_t* vals_out, int tile_count,
+                                         int shift, xbits_t xb, offs_t& offs, int64_t out_limit,
+                                         radix_scatter_storage_t<NT, VT, key_t>& s,
+                                         uint8_t* digits_out = nullptr, int next_shift = 0) {
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int warp = tid / warp_size;
+  const uint32_t last = (uint32_t)tile_count - 1;
+  const uint32_t chunk = (uint32_t)(warp * warp_size * VT) + (uint32_t)lane;
+  key_t k[VT];
+  uint32_t r[VT];
+  uint32_t total, off_local;
+  cta_radix_rank<NT, VT, key_t, full>(keys_in, tile_count, shift, xb, offs, s, k, r, total,
+                                      off_local);
 
   // Values are loaded only now: their latency overlaps the key reorder and
   // stores, and they are not live during the ranking.
@@ -350,7 +370,7 @@ __device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* v
 
 template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
           typename xbits_t>
-__device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
+__device__ __forceinline__ void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
                                          key_t* keys_out, uint32_t* vals_out, int tile_count,
                                          int shift, xbits_t xb, uint32_t goff, int64_t out_limit,
                                          radix_scatter_storage_t<NT, VT, key_t>& s,

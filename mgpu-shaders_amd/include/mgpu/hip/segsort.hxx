@@ -72,7 +72,7 @@ struct segsort_bucket_caps_t { int cap[segsort_max_buckets]; int num_buckets; };
 // Segment i is [begin, end) with both ends clamped to [0, count] and
 // end >= begin: a start at or beyond count gives an empty segment, as in the
 // segmented scan, and no start can place a segment outside the arrays.
-__device__ inline void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
+__device__ __forceinline__ void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
                                       int64_t i, int64_t& begin, int64_t& end) {
   begin = min_<int64_t>(max_<int64_t>(segments[i], 0), count);
   const int64_t next = i + 1 < num_segments ? (int64_t)segments[i + 1] : count;
