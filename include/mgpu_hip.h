@@ -256,7 +256,7 @@ MGPU_DECL_SEGSCAN(f32, float)
  * Tuning: `mgpu_segsort_tuning_t` as returned by mgpu_segmented_sort_autotune
  * (measured on this device) or NULL for the library's current table (the
  * default until an autotune of the same key/value/comparator runs).          */
-#define MGPU_SEGSORT_MAX_BUCKETS 6
+#define MGPU_SEGSORT_MAX_BUCKETS 13
 typedef struct mgpu_segsort_tuning_t {
   int num_buckets;
   int cap[MGPU_SEGSORT_MAX_BUCKETS];   /* bucket b holds lengths <= cap[b]    */

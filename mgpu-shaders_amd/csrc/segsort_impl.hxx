@@ -23,7 +23,9 @@ segsort_tuning_t current_tuning(int has_values, int comp) {
   segsort_tables_t& t = seg_tables();
   std::lock_guard<std::mutex> lock(t.mutex);
   auto it = t.tables.find({has_values, comp});
-  return it == t.tables.end() ? segsort_default_tuning() : it->second;
+  if (it != t.tables.end()) return it->second;
+  return has_values ? segsort_default_tuning<MGPU_SEGSORT_KT, uint32_t>()
+                    : segsort_default_tuning<MGPU_SEGSORT_KT, empty_t>();
 }
 segsort_tuning_t to_tuning(const mgpu_segsort_tuning_t* t, int has_values, int comp) {
   if (!t) return current_tuning(has_values, comp);

@@ -33,19 +33,19 @@ enum gen_kind_t : int {
   gen_sorted_f32 = 9,
 };
 
-__host__ __device__ __forceinline__ uint64_t splitmix64_mix(uint64_t z) {
+__host__ __device__ inline uint64_t splitmix64_mix(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
-__host__ __device__ __forceinline__ uint32_t gen_word32(uint64_t seed, uint64_t stream, uint64_t i) {
+__host__ __device__ inline uint32_t gen_word32(uint64_t seed, uint64_t stream, uint64_t i) {
   uint64_t c = (seed << 40) + (stream << 36) + i + 0x9E3779B97F4A7C15ull;
   return (uint32_t)(splitmix64_mix(c) >> 32);
 }
 
 // Element i of a generator kind, as raw bits (4 or 8 bytes); n is the
 // length of the generated array (used by the sorted kinds).
-__host__ __device__ __forceinline__ uint64_t gen_element(int kind, uint64_t seed, uint64_t i, uint64_t n = 1) {
+__host__ __device__ inline uint64_t gen_element(int kind, uint64_t seed, uint64_t i, uint64_t n = 1) {
   switch (kind) {
     case gen_sorted_u32: {
       uint64_t s = n ? (((uint64_t)1 << 32) / n) : 1;

@@ -29,7 +29,7 @@ enum bounds_t { bounds_lower, bounds_upper };
 // Number of A elements among the first `diag` outputs of the merge of sorted
 // a and b whose ties follow `bounds`: a binary search over the diagonal.
 template <bounds_t bounds = bounds_lower, typename index_t, typename key_t, typename comp_t>
-__host__ __device__ __forceinline__ index_t merge_path_search(const key_t* a, index_t a_count,
+__host__ __device__ inline index_t merge_path_search(const key_t* a, index_t a_count,
                                                      const key_t* b, index_t b_count,
                                                      index_t diag, comp_t comp) {
   index_t begin = diag > b_count ? diag - b_count : 0;
@@ -67,7 +67,7 @@ struct merge4_state_t {
 };
 
 // The state that knows only the run lengths.
-__host__ __device__ __forceinline__ merge4_state_t merge4_initial(int64_t n0, int64_t n1, int64_t n2,
+__host__ __device__ inline merge4_state_t merge4_initial(int64_t n0, int64_t n1, int64_t n2,
                                                          int64_t n3, int64_t diag) {
   const int64_t nx = n0 + n1, ny = n2 + n3;
   return merge4_state_t{max_<int64_t>(0, diag - ny), min_(diag, nx), 0, 0, nx, n0, 0, 0, ny, n2};
@@ -82,7 +82,7 @@ __host__ __device__ __forceinline__ merge4_state_t merge4_initial(int64_t n0, in
 constexpr int merge4_probes = 1;
 
 template <typename key_t, typename comp_t, int P = merge4_probes>
-__host__ __device__ __forceinline__ void merge4_search(const key_t* r0, int64_t n0, const key_t* r1,
+__host__ __device__ inline void merge4_search(const key_t* r0, int64_t n0, const key_t* r1,
                                               int64_t n1, const key_t* r2, int64_t n2,
                                               const key_t* r3, int64_t n3, int64_t diag,
                                               comp_t comp, merge4_state_t st, int64_t (&split)[4]) {
@@ -186,7 +186,7 @@ __host__ __device__ __forceinline__ void merge4_search(const key_t* r0, int64_t 
 }
 
 template <typename key_t, typename comp_t>
-__host__ __device__ __forceinline__ void merge4_path(const key_t* r0, int64_t n0, const key_t* r1,
+__host__ __device__ inline void merge4_path(const key_t* r0, int64_t n0, const key_t* r1,
                                             int64_t n1, const key_t* r2, int64_t n2,
                                             const key_t* r3, int64_t n3, int64_t diag,
                                             comp_t comp, int64_t (&split)[4]) {
@@ -198,7 +198,7 @@ __host__ __device__ __forceinline__ void merge4_path(const key_t* r0, int64_t n0
 // range) is false, with P probes per round; two such searches run
 // interleaved in kary_search2.
 template <int P = merge4_probes, typename pred_t>
-__host__ __device__ __forceinline__ int64_t kary_search(int64_t begin, int64_t end, pred_t pred) {
+__host__ __device__ inline int64_t kary_search(int64_t begin, int64_t end, pred_t pred) {
   while (begin < end) {
     const int64_t w = end - begin;
     int64_t mid[P];
@@ -220,7 +220,7 @@ __host__ __device__ __forceinline__ int64_t kary_search(int64_t begin, int64_t e
 }
 
 template <int P = merge4_probes, typename pred1_t, typename pred2_t>
-__host__ __device__ __forceinline__ void kary_search2(int64_t& b1, int64_t e1, pred1_t pred1, int64_t& b2,
+__host__ __device__ inline void kary_search2(int64_t& b1, int64_t e1, pred1_t pred1, int64_t& b2,
                                              int64_t e2, pred2_t pred2) {
   while (b1 < e1 || b2 < e2) {
     const bool s1 = b1 < e1, s2 = b2 < e2;
@@ -258,7 +258,7 @@ __host__ __device__ __forceinline__ void kary_search2(int64_t& b1, int64_t e1, p
 
 // Per-thread merge path (binary search), used in LDS.
 template <typename key_t, typename comp_t>
-__device__ __forceinline__ int merge_path_lds(const key_t* a, int a_count, const key_t* b,
+__device__ inline int merge_path_lds(const key_t* a, int a_count, const key_t* b,
                                      int b_count, int diag, comp_t comp) {
   return merge_path_search<bounds_lower, int>(a, a_count, b, b_count, diag, comp);
 }
@@ -266,7 +266,7 @@ __device__ __forceinline__ int merge_path_lds(const key_t* a, int a_count, const
 // Merge path in global memory, one binary search per thread (the upper levels
 // of every search touch the same few cache lines, which stay in L2).
 template <bounds_t bounds = bounds_lower, typename key_t, typename comp_t>
-__device__ __forceinline__ int64_t merge_path_global(const key_t* a, int64_t a_count, const key_t* b,
+__device__ inline int64_t merge_path_global(const key_t* a, int64_t a_count, const key_t* b,
                                             int64_t b_count, int64_t diag, comp_t comp) {
   return merge_path_search<bounds, int64_t>(a, a_count, b, b_count, diag, comp);
 }
@@ -277,7 +277,7 @@ __device__ __forceinline__ int64_t merge_path_global(const key_t* a, int64_t a_c
 // caller guarantees that neither range runs out within the VT steps (both
 // hold at least VT keys), and the range tests are dropped.
 template <int VT, bool checked = true, typename key_t, typename comp_t>
-__device__ __forceinline__ void serial_merge(const key_t* keys_shared, int a_begin, int a_end,
+__device__ inline void serial_merge(const key_t* keys_shared, int a_begin, int a_end,
                                     int b_begin, int b_end, key_t (&keys)[VT],
                                     int (&indices)[VT], comp_t comp) {
   key_t a_key = keys_shared[a_begin];
@@ -301,7 +301,7 @@ __device__ __forceinline__ void serial_merge(const key_t* keys_shared, int a_beg
 // at least VT keys left in both ranges (a wave-uniform choice, so the wave
 // runs one of the two loops, never both).
 template <int VT, typename key_t, typename comp_t>
-__device__ __forceinline__ void serial_merge_wave(const key_t* keys_shared, int a_begin, int a_end,
+__device__ inline void serial_merge_wave(const key_t* keys_shared, int a_begin, int a_end,
                                          int b_begin, int b_end, key_t (&keys)[VT],
                                          int (&indices)[VT], comp_t comp) {
   const bool roomy = a_end - a_begin >= VT && b_end - b_begin >= VT;
@@ -366,7 +366,7 @@ struct merge_storage_t {
 // Merge A[a_begin, a_begin+a_cnt) and B[b_begin, b_begin+b_cnt) (a_cnt+b_cnt <=
 // NT*VT) into C[c_begin, ...), keys and values.  Called by the whole block.
 template <int NT, int VT, typename key_t, typename val_t, typename comp_t>
-__device__ __forceinline__ void cta_merge_tile(
+__device__ inline void cta_merge_tile(
     const key_t* a_keys, const val_t* a_vals, int64_t a_begin, int a_cnt,
     const key_t* b_keys, const val_t* b_vals, int64_t b_begin, int b_cnt,
     key_t* c_keys, val_t* c_vals, int64_t c_begin, int c_count, comp_t comp,

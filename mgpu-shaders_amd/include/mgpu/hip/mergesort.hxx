@@ -28,7 +28,7 @@ struct merge_range_t {
 };
 
 // common/cta_mergesort.hxx:17-31.
-__host__ __device__ __forceinline__ merge_range_t compute_mergesort_frame(int64_t partition, int64_t coop,
+__host__ __device__ inline merge_range_t compute_mergesort_frame(int64_t partition, int64_t coop,
                                                                  int64_t spacing) {
   int64_t size = spacing * (coop / 2);
   int64_t start = ~(coop - 1) & partition;
@@ -38,7 +38,7 @@ __host__ __device__ __forceinline__ merge_range_t compute_mergesort_frame(int64_
 }
 
 // common/cta_mergesort.hxx:33-43.
-__host__ __device__ __forceinline__ merge_range_t compute_mergesort_range(int64_t count, int64_t partition,
+__host__ __device__ inline merge_range_t compute_mergesort_range(int64_t count, int64_t partition,
                                                                  int64_t coop, int64_t spacing) {
   merge_range_t f = compute_mergesort_frame(partition, coop, spacing);
   return merge_range_t{f.a_begin, min_(count, f.a_end), min_(count, f.b_begin),
@@ -48,7 +48,7 @@ __host__ __device__ __forceinline__ merge_range_t compute_mergesort_range(int64_
 // common/cta_mergesort.hxx:45-68: the range of partition `partition` given its
 // two merge-path splits; the last partition of a coop group keeps the group's
 // end because its mp1 belongs to the next group's coordinates.
-__host__ __device__ __forceinline__ merge_range_t compute_mergesort_range(int64_t count, int64_t partition,
+__host__ __device__ inline merge_range_t compute_mergesort_range(int64_t count, int64_t partition,
                                                                  int64_t coop, int64_t spacing,
                                                                  int64_t mp0, int64_t mp1) {
   merge_range_t range = compute_mergesort_range(count, partition, coop, spacing);
@@ -76,7 +76,7 @@ struct blocksort_storage_t {
 // registers into an exec-masked branch with copies on both sides.
 template <int VT, bool has_values, bool masked = true, typename key_t, typename val_t,
           typename comp_t>
-__device__ __forceinline__ void odd_even_sort(key_t (&k)[VT], val_t (&v)[VT], uint32_t oor, comp_t comp) {
+__device__ inline void odd_even_sort(key_t (&k)[VT], val_t (&v)[VT], uint32_t oor, comp_t comp) {
 #pragma unroll
   for (int I = 0; I < VT; ++I) {
 #pragma unroll
@@ -98,7 +98,7 @@ __device__ __forceinline__ void odd_even_sort(key_t (&k)[VT], val_t (&v)[VT], ui
 
 // One in-LDS merge pass of the blocksort (common/cta_mergesort.hxx:87-113).
 template <int NT, int VT, bool has_values, typename key_t, typename val_t, typename comp_t>
-__device__ __forceinline__ void block_merge_pass(key_t (&k)[VT], val_t (&v)[VT], int tid, int count,
+__device__ inline void block_merge_pass(key_t (&k)[VT], val_t (&v)[VT], int tid, int count,
                                         int pass, comp_t comp,
                                         blocksort_storage_t<NT, VT, key_t, val_t>& s) {
   const int coop = 2 << pass;
@@ -130,7 +130,7 @@ __device__ __forceinline__ void block_merge_pass(key_t (&k)[VT], val_t (&v)[VT],
 // index_base + position (common/kernel_mergesort.hxx:43-49).  keys_in may
 // equal keys_out (the tile is fully read before it is written).
 template <int NT, int VT, bool sort_indices, typename key_t, typename val_t, typename comp_t>
-__device__ __forceinline__ void cta_blocksort(const key_t* keys_in, const val_t* vals_in, key_t* keys_out,
+__device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in, key_t* keys_out,
                                      val_t* vals_out, int tile_count, int64_t index_base,
                                      comp_t comp, blocksort_storage_t<NT, VT, key_t, val_t>& s) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
@@ -298,7 +298,7 @@ inline bool mergesort_four_way(int64_t count) {
 }
 
 // Samples per side (X or Y) of one group of four runs of `run` keys.
-__host__ __device__ __forceinline__ int64_t merge4_samples_per_group(int64_t run, int64_t spacing) {
+__host__ __device__ inline int64_t merge4_samples_per_group(int64_t run, int64_t spacing) {
   return 2 * run / spacing;
 }
 
@@ -307,7 +307,7 @@ __host__ __device__ __forceinline__ int64_t merge4_samples_per_group(int64_t run
 // r3).  Returns (its merge path within its pair, its position in the group's
 // output), or (0, INT32_MAX) past the end of X or Y.
 template <typename key_t, typename comp_t>
-__host__ __device__ __forceinline__ int2 merge4_sample(const key_t* keys, int64_t count, int64_t run,
+__host__ __device__ inline int2 merge4_sample(const key_t* keys, int64_t count, int64_t run,
                                               int64_t spacing, int64_t per_side, int64_t t,
                                               comp_t comp) {
   const bool y_side = t >= per_side;
@@ -340,7 +340,7 @@ __host__ __device__ __forceinline__ int2 merge4_sample(const key_t* keys, int64_
 }
 
 // Number of samples (ranks ascending, INT32_MAX past the end) ranked below d.
-__host__ __device__ __forceinline__ int64_t merge4_samples_below(const int2* s, int64_t n, int64_t d) {
+__host__ __device__ inline int64_t merge4_samples_below(const int2* s, int64_t n, int64_t d) {
   int64_t begin = 0, end = n;
   while (begin < end) {
     const int64_t mid = (begin + end) >> 1;
@@ -353,7 +353,7 @@ __host__ __device__ __forceinline__ int64_t merge4_samples_below(const int2* s, 
 // Partition c of a four-way pass: the slices of the four runs of the group
 // of position min(c * spacing, count) that precede it.
 template <typename key_t, typename comp_t>
-__host__ __device__ __forceinline__ int4 merge4_partition(const key_t* keys, int64_t count, int64_t run,
+__host__ __device__ inline int4 merge4_partition(const key_t* keys, int64_t count, int64_t run,
                                                  int64_t spacing, int64_t c, const int2* samples,
                                                  int64_t per_side, int64_t sample_spacing,
                                                  comp_t comp) {
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(NT) void kernel_mergesort_partition4(
 // merge of A and B is then merge(p0, p1) followed by merge(p2, p3).  a_split
 // and b_split are the segment boundaries within A and B.
 template <typename key_t, typename comp_t>
-__device__ __forceinline__ int merge_path_seg(const key_t* a, int a_count, int a_split, const key_t* b,
+__device__ inline int merge_path_seg(const key_t* a, int a_count, int a_split, const key_t* b,
                                      int b_count, int b_split, int diag, comp_t comp) {
   int begin = max_(0, diag - b_count), end = min_(diag, a_count);
   while (begin < end) {
@@ -434,7 +434,7 @@ __device__ __forceinline__ int merge_path_seg(const key_t* a, int a_count, int a
 // positions (A in [a_begin, a_end) with segment boundary a_split, B in
 // [b_begin, b_end) with boundary b_split).
 template <int VT, typename key_t, typename comp_t>
-__device__ __forceinline__ void serial_merge_seg(const key_t* keys_shared, int a_begin, int a_end,
+__device__ inline void serial_merge_seg(const key_t* keys_shared, int a_begin, int a_end,
                                         int a_split, int b_begin, int b_end, int b_split,
                                         key_t (&keys)[VT], int (&indices)[VT], comp_t comp) {
   key_t a_key = keys_shared[a_begin];
@@ -457,7 +457,7 @@ __device__ __forceinline__ void serial_merge_seg(const key_t* keys_shared, int a
 
 // Group offset of LDS slot j of a four-way tile (regions p0 p2 p1 p3 ending
 // at q1, q2, q3, each with its own offset o).
-__device__ __forceinline__ uint32_t merge4_slot_offset(int j, int q1, int q2, int q3, int o0, int o2,
+__device__ inline uint32_t merge4_slot_offset(int j, int q1, int q2, int q3, int o0, int o2,
                                               int o1, int o3) {
   return (uint32_t)(j + (j < q1 ? o0 : j < q2 ? o2 : j < q3 ? o1 : o3));
 }

@@ -351,10 +351,10 @@ inline mgpu_status_t init() {
 ////////////////////////////////////////////////////////////////////////////////
 // wave64 primitives.
 
-__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+__device__ inline int lane_id() { return (int)__lane_id(); }
 
 // Number of set bits of `mask` in lanes below the calling lane.
-__device__ __forceinline__ uint32_t mask_count_below(uint64_t mask) {
+__device__ inline uint32_t mask_count_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -362,7 +362,7 @@ __device__ __forceinline__ uint32_t mask_count_below(uint64_t mask) {
 // Inclusive wave scan of a 32-bit value with DPP: row_shr 1,2,4,8 inside each
 // row of 16 lanes, then row_bcast:15 and row_bcast:31 carry across rows.
 // Lanes whose DPP source is out of range read the identity 0.
-__device__ __forceinline__ uint32_t wave_scan_incl_u32(uint32_t x) {
+__device__ inline uint32_t wave_scan_incl_u32(uint32_t x) {
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
@@ -371,7 +371,7 @@ __device__ __forceinline__ uint32_t wave_scan_incl_u32(uint32_t x) {
   x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
   return x;
 }
-__device__ __forceinline__ float wave_scan_incl_f32(float x) {
+__device__ inline float wave_scan_incl_f32(float x) {
 #define MGPU_DPP_F(ctrl, rmask) \
   __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), ctrl, rmask, 0xf, false))
   x += MGPU_DPP_F(0x111, 0xf);
@@ -383,19 +383,19 @@ __device__ __forceinline__ float wave_scan_incl_f32(float x) {
 #undef MGPU_DPP_F
   return x;
 }
-__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) { return wave_scan_incl_u32(x); }
-__device__ __forceinline__ float wave_scan_incl(float x) { return wave_scan_incl_f32(x); }
+__device__ inline uint32_t wave_scan_incl(uint32_t x) { return wave_scan_incl_u32(x); }
+__device__ inline float wave_scan_incl(float x) { return wave_scan_incl_f32(x); }
 
 // Value of the previous lane (lane 0 receives `first`).
 template <typename T>
-__device__ __forceinline__ T wave_shift_up1(T x, T first) {
+__device__ inline T wave_shift_up1(T x, T first) {
   T y = __shfl_up(x, 1, warp_size);
   return lane_id() == 0 ? first : y;
 }
 
 // Butterfly reduction over the wave (every lane receives the total).
 template <typename T>
-__device__ __forceinline__ T wave_reduce_all(T x) {
+__device__ inline T wave_reduce_all(T x) {
 #pragma unroll
   for (int offset = 1; offset < warp_size; offset <<= 1)
     x += __shfl_xor(x, offset, warp_size);
@@ -404,7 +404,7 @@ __device__ __forceinline__ T wave_reduce_all(T x) {
 
 // Lanes of the wave whose 8-bit digit equals the caller's (match_any over
 // eight ballots), restricted to lanes in `active`.
-__device__ __forceinline__ uint64_t match_digit8(uint32_t digit, uint64_t active) {
+__device__ inline uint64_t match_digit8(uint32_t digit, uint64_t active) {
   uint64_t m = active;
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
@@ -421,15 +421,15 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t digit, uint64_t active
 // input fallback (a tile summing a predecessor's input while that tile may
 // overwrite it with output): see status_store_complete / acquire_agent.
 template <typename T>
-__device__ __forceinline__ T atomic_load_agent(const T* p) {
+__device__ inline T atomic_load_agent(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ void atomic_store_agent(T* p, T v) {
+__device__ inline void atomic_store_agent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
-__device__ __forceinline__ T atomic_add_agent(T* p, T v) {
+__device__ inline T atomic_add_agent(T* p, T v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -439,7 +439,7 @@ __device__ __forceinline__ T atomic_add_agent(T* p, T v) {
 // the store is acknowledged, and the signal fence keeps the compiler from
 // moving later stores above it.  (An agent-scope release fence orders them
 // too, but on gfx950 it writes back the L2, once per tile.)
-__device__ __forceinline__ void status_store_complete() {
+__device__ inline void status_store_complete() {
   __builtin_amdgcn_s_waitcnt(0);
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
@@ -447,7 +447,7 @@ __device__ __forceinline__ void status_store_complete() {
 // cache line serves a later load, before its status word is read again.  If
 // any element read was already the predecessor's output, its status store
 // completed before that output was written, so the re-read sees it.
-__device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ inline void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // Zero-fill of 32-bit words (used instead of hipMemsetAsync so the
 // schedule stays capturable in a graph and replays correctly).

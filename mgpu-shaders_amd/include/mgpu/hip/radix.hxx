@@ -40,7 +40,7 @@ constexpr int num_xcds = 8;
 
 // Tile processed by block b when blocks are dealt round-robin to the XCDs:
 // XCD x = b % 8 gets the contiguous range [start(x), start(x) + share(x)).
-__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t num_tiles) {
+__device__ inline int64_t xcd_tile(int64_t b, int64_t num_tiles) {
   const int64_t q = num_tiles / num_xcds, r = num_tiles % num_xcds;
   const int64_t x = b % num_xcds, k = b / num_xcds;
   return x * q + min_(x, r) + k;
@@ -61,7 +61,7 @@ struct radix_bits_t {
 // keep same-address LDS atomics at most 4 deep even when every lane holds the
 // same digit.
 template <int CNT, int NV, typename key_t, typename xbits_t>
-__device__ __forceinline__ void cta_radix_count(const key_t* keys, int tile_count, int shift, xbits_t xb,
+__device__ inline void cta_radix_count(const key_t* keys, int tile_count, int shift, xbits_t xb,
                                        uint32_t* out, int64_t stride, uint32_t* h) {
   constexpr int COLS = 16;
   constexpr int KPV = 16 / (int)sizeof(key_t);   // keys per 128-bit vector
@@ -185,7 +185,7 @@ struct radix_fixed_offsets_t {
 // tile's count of digit d (total) and the tile position of its first key
 // (off_local).  Ends with a block barrier.
 template <int NT, int VT, typename key_t, bool full, typename xbits_t, typename offs_t>
-__device__ __forceinline__ void cta_radix_rank(const key_t* keys_in, int tile_count, int shift, xbits_t xb,
+__device__ inline void cta_radix_rank(const key_t* keys_in, int tile_count, int shift, xbits_t xb,
                                       offs_t& offs, radix_scatter_storage_t<NT, VT, key_t>& s,
                                       key_t (&k)[VT], uint32_t (&r)[VT], uint32_t& total,
                                       uint32_t& off_local) {
@@ -301,7 +301,7 @@ __device__ __forceinline__ void cta_radix_rank(const key_t* keys_in, int tile_co
 
 template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
           typename xbits_t, typename offs_t>
-__device__ __forceinline__ void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
+__device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
                                          key_t* keys_out, uint32_t* vals_out, int tile_count,
                                          int shift, xbits_t xb, offs_t& offs, int64_t out_limit,
                                          radix_scatter_storage_t<NT, VT, key_t>& s,
@@ -370,7 +370,7 @@ __device__ __forceinline__ void cta_radix_scatter(const key_t* keys_in, const ui
 
 template <int NT, int VT, typename key_t, bool has_values, bool full = false, bool emit = false,
           typename xbits_t>
-__device__ __forceinline__ void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
+__device__ inline void cta_radix_scatter(const key_t* keys_in, const uint32_t* vals_in,
                                          key_t* keys_out, uint32_t* vals_out, int tile_count,
                                          int shift, xbits_t xb, uint32_t goff, int64_t out_limit,
                                          radix_scatter_storage_t<NT, VT, key_t>& s,

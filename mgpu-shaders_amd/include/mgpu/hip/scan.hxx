@@ -38,17 +38,17 @@ template <> struct scan_traits<float> { typedef float acc_t; typedef double carr
 // 1 = tile aggregate, 2 = inclusive prefix) in one 64-bit word.
 enum : uint32_t { status_invalid = 0, status_aggregate = 1, status_inclusive = 2 };
 
-__device__ __forceinline__ uint64_t pack_status(uint32_t v, uint32_t flag) {
+__device__ inline uint64_t pack_status(uint32_t v, uint32_t flag) {
   return ((uint64_t)v << 32) | flag;
 }
-__device__ __forceinline__ uint64_t pack_status(double v, uint32_t flag) {
+__device__ inline uint64_t pack_status(double v, uint32_t flag) {
   return ((uint64_t)__double_as_longlong(v) & ~3ull) | flag;
 }
-template <typename carry_t> __device__ __forceinline__ carry_t unpack_status(uint64_t w);
-template <> __device__ __forceinline__ uint32_t unpack_status<uint32_t>(uint64_t w) {
+template <typename carry_t> __device__ inline carry_t unpack_status(uint64_t w);
+template <> __device__ inline uint32_t unpack_status<uint32_t>(uint64_t w) {
   return (uint32_t)(w >> 32);
 }
-template <> __device__ __forceinline__ double unpack_status<double>(uint64_t w) {
+template <> __device__ inline double unpack_status<double>(uint64_t w) {
   return __longlong_as_double((long long)(w & ~3ull));
 }
 
@@ -60,7 +60,7 @@ struct identity_t {
 // Sum of xform(input[first, first + n)) computed by one wave, in the carry
 // type (used when a predecessor tile has not published its aggregate).
 template <typename carry_t, typename acc_t, typename in_t, typename T, typename xform_t>
-__device__ __forceinline__ carry_t wave_tile_sum(const in_t* input, int64_t first, int n, xform_t xform) {
+__device__ inline carry_t wave_tile_sum(const in_t* input, int64_t first, int n, xform_t xform) {
   carry_t sum = 0;
   for (int e = lane_id(); e < n; e += warp_size) sum += (carry_t)(acc_t)(T)xform(input[first + e]);
   return wave_reduce_all(sum);
@@ -75,7 +75,7 @@ __device__ __forceinline__ carry_t wave_tile_sum(const in_t* input, int64_t firs
 // the word is read again after the data: if it is visible now, the computed
 // sum is discarded.
 template <typename carry_t, typename acc_t, int NV, typename in_t, typename T, typename xform_t>
-__device__ __forceinline__ carry_t scan_lookback(const uint64_t* status, int64_t tile, carry_t init,
+__device__ inline carry_t scan_lookback(const uint64_t* status, int64_t tile, carry_t init,
                                         const in_t* input, xform_t xform,
                                         lookback_config_t lb) {
   const int lane = lane_id();
@@ -122,15 +122,15 @@ __device__ __forceinline__ carry_t scan_lookback(const uint64_t* status, int64_t
   return excl;
 }
 
-__device__ __forceinline__ uint32_t carry_combine(uint32_t carry, uint32_t local) { return carry + local; }
-__device__ __forceinline__ float carry_combine(double carry, float local) {
+__device__ inline uint32_t carry_combine(uint32_t carry, uint32_t local) { return carry + local; }
+__device__ inline float carry_combine(double carry, float local) {
   return (float)(carry + (double)local);
 }
 
 // Load tile `tile` (NT*VT elements) as acc_t: 128-bit row-strided loads for a
 // full, aligned tile, guarded scalar loads otherwise.
 template <int NT, int VT, typename acc_t, typename in_t, typename T, typename xform_t>
-__device__ __forceinline__ void scan_load_tile(const in_t* input, int64_t count, int64_t tile, bool vec_ok,
+__device__ inline void scan_load_tile(const in_t* input, int64_t count, int64_t tile, bool vec_ok,
                                       xform_t xform, acc_t (&x)[VT]) {
   constexpr int NV = NT * VT;
   constexpr int NR = VT / 4;
@@ -384,7 +384,7 @@ template <> struct reduce_traits<float> { typedef double acc_t; };
 template <> struct reduce_traits<uint64_t> { typedef uint64_t acc_t; };
 
 template <int NT, typename acc_t>
-__device__ __forceinline__ acc_t block_reduce(acc_t v, acc_t* s_warp) {
+__device__ inline acc_t block_reduce(acc_t v, acc_t* s_warp) {
   constexpr int NW = NT / warp_size;
   v = wave_reduce_all(v);
   int warp = threadIdx.x / warp_size;
@@ -504,7 +504,7 @@ mgpu_status_t reduce(void* aux_data, size_t& aux_size, hipStream_t stream, const
 // so no ordering between the two stores is needed.  8 bytes read and 8
 // written per element.
 
-__device__ __forceinline__ void scan64_publish(uint64_t* status, int64_t tile, uint64_t v, uint32_t flag) {
+__device__ inline void scan64_publish(uint64_t* status, int64_t tile, uint64_t v, uint32_t flag) {
   const uint64_t hi_flag = (uint64_t)flag << 32;
   atomic_store_agent(status + 2 * tile, hi_flag | (v & (uint64_t)0xFFFFFFFFu));
   atomic_store_agent(status + 2 * tile + 1, hi_flag | (v >> 32));
@@ -512,7 +512,7 @@ __device__ __forceinline__ void scan64_publish(uint64_t* status, int64_t tile, u
 
 // Flag of a word pair: the common flag, or 3 for a torn pair.
 constexpr uint32_t status64_torn = 3;
-__device__ __forceinline__ uint32_t scan64_read(const uint64_t* status, int64_t tile, uint64_t& v) {
+__device__ inline uint32_t scan64_read(const uint64_t* status, int64_t tile, uint64_t& v) {
   const uint64_t w0 = atomic_load_agent(status + 2 * tile);
   const uint64_t w1 = atomic_load_agent(status + 2 * tile + 1);
   v = (w1 << 32) | (w0 & (uint64_t)0xFFFFFFFFu);
@@ -525,7 +525,7 @@ __device__ __forceinline__ uint32_t scan64_read(const uint64_t* status, int64_t 
 // missing tile's sum from its input with the in-place re-check.  A torn pair
 // means its owner is between its two stores, so it is only ever re-polled.
 template <int NV, typename T>
-__device__ __forceinline__ uint64_t scan64_lookback(const uint64_t* status, int64_t tile, uint64_t init,
+__device__ inline uint64_t scan64_lookback(const uint64_t* status, int64_t tile, uint64_t init,
                                            const T* input, lookback_config_t lb) {
   const int lane = lane_id();
   uint64_t excl = 0;

@@ -61,13 +61,13 @@ template <int ctrl, int rmask> struct dpp_mover<float, ctrl, rmask> {
   }
 };
 template <typename V, int ctrl, int rmask>
-__device__ __forceinline__ V dpp_move(V v) {
+__device__ inline V dpp_move(V v) {
   return dpp_mover<V, ctrl, rmask>::move(v);
 }
 
 // Inclusive segmented wave scan of (f, v) over 64 lanes (lane order).
 template <typename V>
-__device__ __forceinline__ void wave_segscan_incl(uint32_t& f, V& v) {
+__device__ inline void wave_segscan_incl(uint32_t& f, V& v) {
 #define MGPU_SEG_STEP(ctrl, rmask)                                                       \
   {                                                                                      \
     uint32_t fp = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)f, ctrl, rmask, 0xf,     \
@@ -88,7 +88,7 @@ __device__ __forceinline__ void wave_segscan_incl(uint32_t& f, V& v) {
 // Carry-out contribution of tile jj computed from its input by one wave:
 // (has_head, sum from its last head, or from its first element, to its end).
 template <typename carry_t, typename acc_t, int NV, typename in_t, typename T, typename xform_t>
-__device__ __forceinline__ carry_t segscan_tile_tail(const in_t* input, const int32_t* segments,
+__device__ inline carry_t segscan_tile_tail(const in_t* input, const int32_t* segments,
                                             const int32_t* tile_first, int64_t jj,
                                             xform_t xform, bool& has_head) {
   const int64_t base = jj * NV;
@@ -113,7 +113,7 @@ __device__ __forceinline__ carry_t segscan_tile_tail(const in_t* input, const in
 // in-place scans).  A predecessor whose carry-out is complete on its own
 // (it holds a head) publishes an inclusive word, so the walk stops there.
 template <typename carry_t, typename acc_t, int NV, typename in_t, typename T, typename xform_t>
-__device__ __forceinline__ carry_t segscan_lookback(const uint64_t* status, int64_t tile, carry_t init,
+__device__ inline carry_t segscan_lookback(const uint64_t* status, int64_t tile, carry_t init,
                                            const in_t* input, const int32_t* segments,
                                            const int32_t* tile_first, xform_t xform,
                                            lookback_config_t lb) {
@@ -173,7 +173,7 @@ struct seg_pair_t {
   acc_t v;
 };
 template <typename acc_t>
-__device__ __forceinline__ seg_pair_t<acc_t> seg_combine(seg_pair_t<acc_t> a, seg_pair_t<acc_t> b) {
+__device__ inline seg_pair_t<acc_t> seg_combine(seg_pair_t<acc_t> a, seg_pair_t<acc_t> b) {
   return {a.f | b.f, b.f ? b.v : (acc_t)(a.v + b.v)};
 }
 
@@ -184,19 +184,19 @@ struct seg_cpair_t {
   carry_t v;
 };
 template <typename carry_t>
-__device__ __forceinline__ seg_cpair_t<carry_t> seg_combine_c(seg_cpair_t<carry_t> a, uint32_t bf,
+__device__ inline seg_cpair_t<carry_t> seg_combine_c(seg_cpair_t<carry_t> a, uint32_t bf,
                                                      carry_t bv) {
   return {a.f | bf, bf ? bv : (carry_t)(a.v + bv)};
 }
 
 // Result of one element: `pre` is the pair of everything in the tile before
 // its wave row segment (carry type), `e` its prefix inside the segment.
-__device__ __forceinline__ uint32_t segscan_finish(uint32_t carry, uint32_t init,
+__device__ inline uint32_t segscan_finish(uint32_t carry, uint32_t init,
                                           seg_cpair_t<uint32_t> pre, seg_pair_t<uint32_t> e) {
   if (e.f) return init + e.v;
   return (pre.f ? init : carry) + pre.v + e.v;
 }
-__device__ __forceinline__ float segscan_finish(double carry, float init, seg_cpair_t<double> pre,
+__device__ inline float segscan_finish(double carry, float init, seg_cpair_t<double> pre,
                                        seg_pair_t<float> e) {
   if (e.f) return (float)((double)init + (double)e.v);
   return (float)((pre.f ? (double)init : carry) + pre.v + (double)e.v);

@@ -25,6 +25,11 @@
 //      Descending order uses inverted bits; float keys map -0.0 to +0.0 so
 //      ties follow comparator semantics.
 #pragma once
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <utility>
+
 #include "mergesort.hxx"
 #include "radix.hxx"
 #include "scan.hxx"
@@ -32,19 +37,35 @@
 namespace mgpu {
 namespace hip {
 
-// Blocksort shapes available to the buckets, by capacity.  VT is odd: the
-// blocksort reads and writes LDS in thread order (word VT * tid + i), and an
-// odd stride spreads a wave's accesses over every bank, where a power-of-two
-// VT puts them in a few (measured on MI355X: 2048 tiles of 8192 key-value
-// pairs take 0.51 ms at 512 x 16, 0.15 ms per 2^24 pairs at 512 x 19).
+// Blocksort shapes available to the buckets.  VT is odd: the blocksort reads
+// and writes LDS in thread order (word VT * tid + i), and an odd stride
+// spreads a wave's accesses over every bank, where a power-of-two VT puts
+// them in a few (measured on MI355X: 2048 tiles of 8192 key-value pairs take
+// 0.51 ms at 512 x 16, 0.15 ms per 2^24 pairs at 512 x 19).  Indices 0-7 are
+// the original table; 8-15 were added for buckets spaced 1.5x apart.
 struct segsort_shape_t { int nt, vt; };
-constexpr int segsort_num_shapes = 8;
+constexpr int segsort_num_shapes = 16;
 constexpr segsort_shape_t segsort_shapes[segsort_num_shapes] = {
-    {64, 5}, {64, 9}, {128, 9}, {64, 17}, {256, 9}, {256, 17}, {512, 9}, {512, 17}};
-constexpr int segsort_max_buckets = 6;
+    {64, 5},   {64, 9},   {128, 9},  {64, 17},  {256, 9},  {256, 17}, {512, 9},  {512, 17},
+    {64, 3},   {64, 7},   {128, 7},  {128, 13}, {128, 17}, {256, 13}, {512, 13}, {1024, 17}};
+constexpr int segsort_max_buckets = 13;
 // Large-segment radix tiles.
 constexpr int segsort_radix_nt = 512, segsort_radix_vt = 16, segsort_radix_cnt = 1024;
 constexpr int segsort_radix_nv = segsort_radix_nt * segsort_radix_vt;
+
+// Whether shape i's blocksort tile fits the 160 KB of LDS for these key and
+// value types (1024 x 17 holds 4-byte keys with values, or 8-byte keys alone).
+template <typename key_t, typename val_t, int... I>
+constexpr std::array<bool, sizeof...(I)> segsort_fit_table(std::integer_sequence<int, I...>) {
+  return {{(sizeof(blocksort_storage_t<segsort_shapes[I].nt, segsort_shapes[I].vt, key_t,
+                                       val_t>) <= 160 * 1024)...}};
+}
+template <typename key_t, typename val_t>
+inline bool segsort_shape_fits(int shape) {
+  constexpr std::array<bool, segsort_num_shapes> fit =
+      segsort_fit_table<key_t, val_t>(std::make_integer_sequence<int, segsort_num_shapes>());
+  return shape >= 0 && shape < segsort_num_shapes && fit[shape];
+}
 
 // A bucket table: buckets hold segments of length <= cap[b] (ascending); each
 // bucket is sorted by blocksort shape `shape[b]` (index into segsort_shapes,
@@ -55,16 +76,23 @@ struct segsort_tuning_t {
   int shape[segsort_max_buckets];
 };
 
-inline segsort_tuning_t segsort_default_tuning() {
+// Capacities 1.5x apart, so that a segment fills about four fifths of its
+// tile on average (at 2x apart, about two thirds), up to 16384 keys where
+// the 1024 x 17 tile fits the LDS and 8192 otherwise.
+inline segsort_tuning_t segsort_tuning_for(bool tile_16k) {
   segsort_tuning_t t;
-  t.num_buckets = 6;
-  const int caps[6] = {256, 512, 1024, 2048, 4096, 8192};
-  const int shapes[6] = {0, 1, 2, 4, 5, 7};
-  for (int b = 0; b < 6; ++b) {
-    t.cap[b] = caps[b];
-    t.shape[b] = shapes[b];
+  const int caps[13] = {128, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 16384};
+  const int shapes[13] = {8, 0, 9, 1, 10, 2, 11, 4, 13, 5, 14, 7, 15};
+  t.num_buckets = tile_16k ? 13 : 12;
+  for (int b = 0; b < segsort_max_buckets; ++b) {
+    t.cap[b] = b < t.num_buckets ? caps[b] : 0;
+    t.shape[b] = b < t.num_buckets ? shapes[b] : 0;
   }
   return t;
+}
+template <typename key_t, typename val_t>
+inline segsort_tuning_t segsort_default_tuning() {
+  return segsort_tuning_for(segsort_shape_fits<key_t, val_t>(15));
 }
 
 struct segsort_bucket_caps_t { int cap[segsort_max_buckets]; int num_buckets; };
@@ -72,7 +100,7 @@ struct segsort_bucket_caps_t { int cap[segsort_max_buckets]; int num_buckets; };
 // Segment i is [begin, end) with both ends clamped to [0, count] and
 // end >= begin: a start at or beyond count gives an empty segment, as in the
 // segmented scan, and no start can place a segment outside the arrays.
-__device__ __forceinline__ void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
+__device__ inline void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
                                       int64_t i, int64_t& begin, int64_t& end) {
   begin = min_<int64_t>(max_<int64_t>(segments[i], 0), count);
   const int64_t next = i + 1 < num_segments ? (int64_t)segments[i + 1] : count;
@@ -268,10 +296,14 @@ inline hipError_t segsort_launch_bucket(key_t* keys, val_t* vals, int64_t count,
                                         const int32_t* list, const int32_t* bounds, comp_t comp,
                                         hipStream_t stream) {
   constexpr int NT = segsort_shapes[SHAPE].nt, VT = segsort_shapes[SHAPE].vt;
-  const unsigned grid = (unsigned)min_<int64_t>(num_segments, segsort_grid_threads / NT);
-  return launch_kernel("mgpu_segsort_blocks", kernel_segsort_blocks<NT, VT, key_t, val_t, comp_t>,
-                       dim3(grid), dim3(NT), 0, stream, keys, vals, count, segments,
-                       num_segments, list, bounds, comp);
+  if constexpr (sizeof(blocksort_storage_t<NT, VT, key_t, val_t>) <= 160 * 1024) {
+    const unsigned grid = (unsigned)min_<int64_t>(num_segments, segsort_grid_threads / NT);
+    return launch_kernel("mgpu_segsort_blocks",
+                         kernel_segsort_blocks<NT, VT, key_t, val_t, comp_t>, dim3(grid), dim3(NT),
+                         0, stream, keys, vals, count, segments, num_segments, list, bounds, comp);
+  } else {
+    return hipErrorInvalidValue;  // segmented_sort rejects such a table first
+  }
 }
 
 template <typename key_t, typename val_t, typename comp_t>
@@ -286,6 +318,8 @@ inline hipError_t segsort_dispatch_shape(int shape, key_t* keys, val_t* vals, in
   switch (shape) {
     MGPU_SEGSORT_SHAPE(0) MGPU_SEGSORT_SHAPE(1) MGPU_SEGSORT_SHAPE(2) MGPU_SEGSORT_SHAPE(3)
     MGPU_SEGSORT_SHAPE(4) MGPU_SEGSORT_SHAPE(5) MGPU_SEGSORT_SHAPE(6) MGPU_SEGSORT_SHAPE(7)
+    MGPU_SEGSORT_SHAPE(8) MGPU_SEGSORT_SHAPE(9) MGPU_SEGSORT_SHAPE(10) MGPU_SEGSORT_SHAPE(11)
+    MGPU_SEGSORT_SHAPE(12) MGPU_SEGSORT_SHAPE(13) MGPU_SEGSORT_SHAPE(14) MGPU_SEGSORT_SHAPE(15)
   }
 #undef MGPU_SEGSORT_SHAPE
   return hipErrorInvalidValue;
@@ -302,7 +336,7 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   if (count < 0 || num_segments < 0) return MGPU_INVALID_ARG;
   if (tuning.num_buckets < 1 || tuning.num_buckets > segsort_max_buckets) return MGPU_INVALID_ARG;
   for (int b = 0; b < tuning.num_buckets; ++b) {
-    if (tuning.shape[b] < 0 || tuning.shape[b] >= segsort_num_shapes) return MGPU_INVALID_ARG;
+    if (!segsort_shape_fits<key_t, val_t>(tuning.shape[b])) return MGPU_INVALID_ARG;
     const segsort_shape_t sh = segsort_shapes[tuning.shape[b]];
     if (tuning.cap[b] < 2 || tuning.cap[b] > sh.nt * sh.vt) return MGPU_INVALID_ARG;
     if (b && tuning.cap[b] <= tuning.cap[b - 1]) return MGPU_INVALID_ARG;
@@ -409,10 +443,19 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
 }
 
 ////////////////////////////////////////////////////////////////////////////////
-// Autotuner: for every size class (cap[b-1], cap[b]] of the default bucket
-// table, time each blocksort shape that can hold cap[b] on synthetic
-// segments with lengths uniform over the class (SplitMix64 keys), and keep
-// the fastest.  Returns the table (also usable as an explicit argument).
+// Autotuner: picks the blocksort shape of every bucket of the default table
+// by coordinate descent over whole segmented-sort calls.  The buckets of one
+// call run concurrently and share the CUs, so a shape that is fastest alone
+// (large NT, high occupancy, many empty slots) is not the fastest in the
+// call: measured on 65,536 lognormal segments, the shapes chosen by timing
+// each bucket alone made the call 8% slower than the default table.  Here the
+// trial is the whole call on synthetic segments (about 2^22 SplitMix64
+// keys, the same number of keys in every octave of length from 2 to twice
+// the largest capacity); each bucket in turn tries every shape that holds
+// its capacity (up to 8x it) with the other buckets fixed, keeping the
+// fastest (median of 3), two rounds.  Returns the table (also usable as an explicit argument);
+// ms_table[b][shape] is the call's time with that shape in bucket b (last
+// round), -1 for shapes not tried.
 
 template <typename key_t>
 __global__ __launch_bounds__(256) void kernel_segsort_tune_fill(key_t* keys, uint32_t* vals,
@@ -432,31 +475,15 @@ template <typename key_t, typename val_t, typename comp_t>
 inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_tuning_t& out,
                                       double* ms_table /* [buckets][shapes] or null */) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
-  out = segsort_default_tuning();
-  const int64_t target = (int64_t)1 << 22;  // keys per trial
-  key_t* keys = nullptr;
-  uint32_t* vals = nullptr;
-  int32_t* segs = nullptr;
-  int32_t* list = nullptr;
-  int32_t* bounds = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  mgpu_status_t status = MGPU_OK;
-  auto check = [&](hipError_t e) {
-    if (e != hipSuccess && status == MGPU_OK) status = hip_status(e);
-    return e == hipSuccess;
-  };
-  const int64_t max_segs = target / 2 + 1;
-  if (!check(hipMalloc(&keys, target * sizeof(key_t))) ||
-      !check(hipMalloc(&vals, target * sizeof(uint32_t))) ||
-      !check(hipMalloc(&segs, max_segs * sizeof(int32_t))) ||
-      !check(hipMalloc(&list, max_segs * sizeof(int32_t))) ||
-      !check(hipMalloc(&bounds, 2 * sizeof(int32_t))) || !check(hipEventCreate(&e0)) ||
-      !check(hipEventCreate(&e1))) {
-    (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(segs); (void)hipFree(list);
-    (void)hipFree(bounds);
-    return status;
-  }
-  std::vector<int32_t> h_segs, h_list;
+  constexpr bool descending = std::is_same_v<comp_t, greater_t<key_t>>;
+  out = segsort_default_tuning<key_t, val_t>();
+  if (ms_table)
+    for (int i = 0; i < segsort_max_buckets * segsort_num_shapes; ++i) ms_table[i] = -1;
+  // Synthetic layout: lengths in every octave [2^k, 2^(k+1)) from 2 up to
+  // twice the largest capacity, the same number of keys in each octave
+  // (lengths uniform within it), segments in a shuffled order.
+  const int64_t target = (int64_t)1 << 22;
+  const int top = 2 * out.cap[out.num_buckets - 1];
   uint64_t rng = 0x243F6A8885A308D3ull;
   auto next_rand = [&]() {
     rng += 0x9E3779B97F4A7C15ull;
@@ -465,50 +492,95 @@ inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_t
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
   };
-  for (int b = 0; b < out.num_buckets && status == MGPU_OK; ++b) {
-    const int lo = b ? out.cap[b - 1] + 1 : 2, hi = out.cap[b];
-    h_segs.clear();
-    int64_t n = 0;
-    while (n < target - hi) {
-      h_segs.push_back((int32_t)n);
-      n += lo + (int64_t)(next_rand() % (uint64_t)(hi - lo + 1));
-    }
-    const int64_t S = (int64_t)h_segs.size();
-    h_list.resize(S);
-    for (int64_t i = 0; i < S; ++i) h_list[i] = (int32_t)i;
-    const int32_t h_bounds[2] = {0, (int32_t)S};
-    check(hipMemcpy(segs, h_segs.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
-    check(hipMemcpy(list, h_list.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
-    check(hipMemcpy(bounds, h_bounds, sizeof(h_bounds), hipMemcpyHostToDevice));
-    double best = 1e30;
-    for (int sh = 0; sh < segsort_num_shapes && status == MGPU_OK; ++sh) {
-      const int capacity = segsort_shapes[sh].nt * segsort_shapes[sh].vt;
-      double ms = -1;
-      if (capacity >= hi && capacity <= 8 * hi) {
-        float total = 0;
-        for (int rep = 0; rep < 3; ++rep) {
-          hipLaunchKernelGGL(kernel_segsort_tune_fill<key_t>, dim3(1024), dim3(256), 0, stream, keys,
-                             has_values ? vals : nullptr, n, (uint64_t)rep);
-          check(hipEventRecord(e0, stream));
-          check(segsort_dispatch_shape(sh, keys, has_values ? (val_t*)vals : (val_t*)nullptr, n,
-                                       segs, S, list, bounds, comp, stream));
-          check(hipEventRecord(e1, stream));
-          check(hipEventSynchronize(e1));
-          float t = 0;
-          check(hipEventElapsedTime(&t, e0, e1));
-          if (rep) total += t;  // the first trial warms up
-        }
-        ms = total / 2;
-        if (ms < best) {
-          best = ms;
-          out.shape[b] = sh;
-        }
-      }
-      if (ms_table) ms_table[b * segsort_num_shapes + sh] = ms;
+  std::vector<int32_t> lens;
+  int octaves = 0;
+  for (int lo = 2; lo < top; lo *= 2) ++octaves;
+  for (int lo = 2; lo < top; lo *= 2) {
+    const int hi = min_(2 * lo, top);
+    for (int64_t k = 0; k < target / octaves;) {
+      const int len = lo + (int)(next_rand() % (uint64_t)(hi - lo));
+      lens.push_back(len);
+      k += len;
     }
   }
-  (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(segs); (void)hipFree(list);
-  (void)hipFree(bounds); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+  for (size_t i = lens.size(); i > 1; --i) std::swap(lens[i - 1], lens[next_rand() % i]);
+  std::vector<int32_t> h_segs(lens.size());
+  int64_t n = 0;
+  for (size_t i = 0; i < lens.size(); ++i) {
+    h_segs[i] = (int32_t)n;
+    n += lens[i];
+  }
+  const int64_t S = (int64_t)h_segs.size();
+  key_t* keys = nullptr;
+  uint32_t* vals = nullptr;
+  int32_t* segs = nullptr;
+  void* aux = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  mgpu_status_t status = MGPU_OK;
+  auto check = [&](hipError_t e) {
+    if (e != hipSuccess && status == MGPU_OK) status = hip_status(e);
+    return e == hipSuccess;
+  };
+  // Aux for any table of this bucket count: the sizing call depends on the
+  // capacities only through the largest one, which the descent keeps.
+  size_t aux_size = 0;
+  segmented_sort<descending>(nullptr, aux_size, stream, (key_t*)nullptr, (val_t*)nullptr, n,
+                             (const int32_t*)nullptr, S, comp, out);
+  if (!check(hipMalloc(&keys, n * sizeof(key_t))) ||
+      !check(hipMalloc(&vals, n * sizeof(uint32_t))) ||
+      !check(hipMalloc(&segs, S * sizeof(int32_t))) || !check(hipMalloc(&aux, aux_size)) ||
+      !check(hipEventCreate(&e0)) || !check(hipEventCreate(&e1))) {
+    (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(segs); (void)hipFree(aux);
+    if (e0) (void)hipEventDestroy(e0);
+    return status;
+  }
+  check(hipMemcpy(segs, h_segs.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
+  // One trial: the median of three timed calls on freshly filled keys.
+  auto trial = [&](const segsort_tuning_t& t) -> double {
+    float ms[3];
+    for (int rep = 0; rep < 4 && status == MGPU_OK; ++rep) {
+      hipLaunchKernelGGL(kernel_segsort_tune_fill<key_t>, dim3(1024), dim3(256), 0, stream, keys,
+                         has_values ? vals : nullptr, n, (uint64_t)rep);
+      check(hipEventRecord(e0, stream));
+      size_t cap = aux_size;
+      const mgpu_status_t st = segmented_sort<descending>(
+          aux, cap, stream, keys, has_values ? (val_t*)vals : (val_t*)nullptr, n, segs, S, comp,
+          t);
+      if (st != MGPU_OK && status == MGPU_OK) status = st;
+      check(hipEventRecord(e1, stream));
+      check(hipEventSynchronize(e1));
+      float x = 0;
+      check(hipEventElapsedTime(&x, e0, e1));
+      if (rep) ms[rep - 1] = x;  // the first call warms up
+    }
+    if (status != MGPU_OK) return 1e30;
+    std::sort(ms, ms + 3);
+    return ms[1];
+  };
+  double best = trial(out);
+  for (int round = 0; round < 2 && status == MGPU_OK; ++round) {
+    for (int b = 0; b < out.num_buckets && status == MGPU_OK; ++b) {
+      const int cap = out.cap[b];
+      for (int sh = 0; sh < segsort_num_shapes && status == MGPU_OK; ++sh) {
+        const int capacity = segsort_shapes[sh].nt * segsort_shapes[sh].vt;
+        if (capacity < cap || capacity > 8 * cap || !segsort_shape_fits<key_t, val_t>(sh))
+          continue;
+        double ms = best;
+        if (sh != out.shape[b]) {
+          segsort_tuning_t t = out;
+          t.shape[b] = sh;
+          ms = trial(t);
+          if (ms < best) {
+            best = ms;
+            out.shape[b] = sh;
+          }
+        }
+        if (ms_table) ms_table[b * segsort_num_shapes + sh] = ms;
+      }
+    }
+  }
+  (void)hipFree(keys); (void)hipFree(vals); (void)hipFree(segs); (void)hipFree(aux);
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
   return status;
 }
 

@@ -93,8 +93,8 @@ def load():
         f = getattr(lib, "mgpu_segmented_scan_%s" % t)
         f.restype, f.argtypes = c_int, [vp, sz, vp, vp, vp, i64, vp, i64, c_int, _CTYPE[t]]
     class SegsortTuning(ctypes.Structure):
-        _fields_ = [("num_buckets", ctypes.c_int), ("cap", ctypes.c_int * 6),
-                    ("shape", ctypes.c_int * 6)]
+        _fields_ = [("num_buckets", ctypes.c_int), ("cap", ctypes.c_int * SEGSORT_MAX_BUCKETS),
+                    ("shape", ctypes.c_int * SEGSORT_MAX_BUCKETS)]
     global _SegsortTuning
     _SegsortTuning = SegsortTuning
     tp = ctypes.POINTER(SegsortTuning)
@@ -412,20 +412,24 @@ def reduce(input, out, init=0, elem_type=None, aux=None, stream=None):
 # Segmented sort (batched; segments[i] = first element of segment i)
 
 _SegsortTuning = None
+SEGSORT_MAX_BUCKETS = 13  # MGPU_SEGSORT_MAX_BUCKETS of mgpu_hip.h
 
 
 def segsort_shapes():
     """[(nt, vt)] of the blocksort shapes a bucket can use."""
     lib = load()
-    nt = (ctypes.c_int * 16)()
-    vt = (ctypes.c_int * 16)()
-    n = lib.mgpu_segsort_shapes(nt, vt, 16)
-    return [(nt[i], vt[i]) for i in range(n)]
+    nt = (ctypes.c_int * 64)()
+    vt = (ctypes.c_int * 64)()
+    n = lib.mgpu_segsort_shapes(nt, vt, 64)
+    return [(nt[i], vt[i]) for i in range(min(n, 64))]
 
 
 def _tuning_struct(tuning):
     if tuning is None:
         return None
+    if len(tuning["cap"]) > SEGSORT_MAX_BUCKETS or len(tuning["shape"]) != len(tuning["cap"]):
+        raise ValueError("a tuning table holds at most %d buckets, one shape per cap"
+                         % SEGSORT_MAX_BUCKETS)
     t = _SegsortTuning()
     t.num_buckets = len(tuning["cap"])
     for b, (c, sh) in enumerate(zip(tuning["cap"], tuning["shape"])):
@@ -457,13 +461,16 @@ def segmented_sort(keys, segments, vals=None, comp="less", key_type=None, tuning
 
 
 def segmented_sort_autotune(key_type="u32", has_values=True, comp="less", stream=None):
-    """Time every blocksort shape per size bucket on this device; the chosen
-    table becomes the default for this key/value/comparator and is returned
-    as {"cap": [...], "shape": [...], "ms": [[...]]}."""
+    """Choose the blocksort shape of every size bucket on this device, by
+    timing whole segmented sorts of a synthetic layout (coordinate descent
+    over the buckets); the chosen table becomes the default for this
+    key/value/comparator and is returned as {"cap": [...], "shape": [...],
+    "ms": [[...]]}, ms[b][i] = the call's time with shape i in bucket b (-1
+    where not tried)."""
     lib = load()
     t = _SegsortTuning()
     nshapes = len(segsort_shapes())
-    ms = (ctypes.c_double * (6 * nshapes))()
+    ms = (ctypes.c_double * (SEGSORT_MAX_BUCKETS * nshapes))()
     _check(getattr(lib, "mgpu_segmented_sort_autotune_%s" % key_type)(
         _stream(stream), 1 if has_values else 0, _comp(comp), ctypes.byref(t), ms),
         "segmented_sort_autotune")

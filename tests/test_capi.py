@@ -229,6 +229,57 @@ def test_misaligned_aux_is_rejected_before_any_launch():
                 mgpu_hip.MGPU_AUX_TOO_SMALL, (fn, base)
 
 
+def test_segsort_tuning_tables_are_checked_before_any_launch():
+    """A bucket table is checked by the sizing call: a blocksort shape whose
+    tile exceeds the 160 KB of LDS for the key and value types (1024 x 17 with
+    8-byte keys and values), more buckets than the table holds, or capacities
+    that do not ascend are argument errors; the default table of each type is
+    accepted.  No pointer is dereferenced."""
+    lib = mgpu_hip.load()
+    seg = ctypes.c_void_p(256)
+    shapes = [(nt, vt) for nt, vt in zip(*_shape_arrays(lib))]
+    big = shapes.index((1024, 17))
+    caps = [128, 256, 384, 512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 16384]
+    sh = [8, 0, 9, 1, 10, 2, 11, 4, 13, 5, 14, 7, big]
+    for i, c in enumerate(caps):
+        assert shapes[sh[i]][0] * shapes[sh[i]][1] >= c
+
+    def table(cs, ss, nb=None):
+        t = mgpu_hip._SegsortTuning()
+        t.num_buckets = len(cs) if nb is None else nb
+        for b, (c, x) in enumerate(zip(cs, ss)):
+            t.cap[b] = c
+            t.shape[b] = x
+        return ctypes.byref(t)
+
+    def size(fn, tuning, kv):
+        args = (ctypes.c_void_p(1),) * (2 if kv else 1) + (1 << 20, seg, 100, 0, tuning)
+        return _size(fn, *args)[0]
+
+    full = table(caps, sh)
+    assert size(lib.mgpu_segmented_sort_kv_u32_u32, full, True) == mgpu_hip.MGPU_OK
+    assert size(lib.mgpu_segmented_sort_keys_u64, full, False) == mgpu_hip.MGPU_OK
+    assert size(lib.mgpu_segmented_sort_kv_u64_u32, full, True) == mgpu_hip.MGPU_INVALID_ARG
+    assert size(lib.mgpu_segmented_sort_kv_f64_u32, full, True) == mgpu_hip.MGPU_INVALID_ARG
+    assert size(lib.mgpu_segmented_sort_kv_u64_u32, table(caps[:12], sh[:12]), True) == \
+        mgpu_hip.MGPU_OK
+    assert size(lib.mgpu_segmented_sort_kv_u32_u32, table(caps, sh, nb=14), True) == \
+        mgpu_hip.MGPU_INVALID_ARG
+    assert size(lib.mgpu_segmented_sort_kv_u32_u32, table(caps[::-1], sh[::-1]), True) == \
+        mgpu_hip.MGPU_INVALID_ARG
+    for fn, kv in [(lib.mgpu_segmented_sort_kv_u32_u32, True),
+                   (lib.mgpu_segmented_sort_kv_u64_u32, True),
+                   (lib.mgpu_segmented_sort_keys_f64, False)]:
+        assert size(fn, None, kv) == mgpu_hip.MGPU_OK
+
+
+def _shape_arrays(lib):
+    nt = (ctypes.c_int * 64)()
+    vt = (ctypes.c_int * 64)()
+    n = lib.mgpu_segsort_shapes(nt, vt, 64)
+    return list(nt[:n]), list(vt[:n])
+
+
 def test_python_surface_rejects_host_tensors():
     import torch
     with pytest.raises(ValueError):

@@ -25,9 +25,12 @@ def make_keys(kt, n, seed, ties=True):
 
 
 def boundary_lengths():
-    """Lengths on both sides of every bucket capacity and blocksort shape
-    capacity, and of one, two and three large-segment radix tiles (8192)."""
-    caps = [256, 320, 512, 576, 1024, 1152, 2048, 2304, 4096, 4352, 8192, 8704, 12288, 16384]
+    """Lengths on both sides of every bucket capacity of the default tables
+    (128 ... 8192, and 16384 where the 1024 x 17 tile fits) and of every
+    blocksort shape capacity, and of one to four large-segment radix tiles
+    (8192)."""
+    caps = [128, 192, 256, 320, 384, 448, 512, 576, 768, 896, 1024, 1152, 1536, 1664, 2048,
+            2304, 3072, 3328, 4096, 4352, 6144, 6656, 8192, 8704, 12288, 16384, 17408]
     lens = [0, 1, 2, 3, 0, 1]
     for c in caps:
         lens += [c - 1, c, c + 1]

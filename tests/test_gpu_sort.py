@@ -257,6 +257,32 @@ def test_radix_kv(n, kt, radix_algo):
     np.testing.assert_array_equal(to_host(dv), ev)
 
 
+# Sorts of 4097 to 4178 tiles of the large-sort shapes (6144 keys per tile for
+# 64-bit keys, 10240 / 12288 for 32-bit key-value / keys), each ending in a
+# partial tile: the 64-bit passes 1-7 count the digit bytes the previous
+# scatter wrote, the 32-bit passes the keys.
+LARGE_TILES = [("u64", 4178 * 256 * 24 - 5196, True), ("f64", 4097 * 256 * 24 - 1, True),
+               ("u32", 4101 * 512 * 20 - 77, True), ("u32", 4097 * 512 * 24 - 1, False),
+               ("u64", 4099 * 512 * 12 - 3, False)]
+
+
+@pytest.mark.parametrize("kt,n,kv", LARGE_TILES)
+def test_radix_large_tile_counts(kt, n, kv):
+    banner("radix large tile counts", kt, n, "kv" if kv else "keys")
+    keys = make_keys(kt, n, 13)
+    dk = dev_keys(keys)
+    if kv:
+        vals = np.arange(n, dtype=np.uint32)
+        dv = to_dev(vals)
+        mgpu_hip.radix_sort(dk, dv)
+        ek, ev = oracle.stable_sort_radix(keys, vals, kt)
+        np.testing.assert_array_equal(to_host(dv), ev)
+    else:
+        mgpu_hip.radix_sort(dk)
+        ek = oracle.stable_sort_radix(keys, None, kt)
+    np.testing.assert_array_equal(to_host(dk).view(np.uint8), ek.view(np.uint8))
+
+
 @pytest.mark.parametrize("pattern", ["random", "equal", "sorted", "reversed", "lowentropy"])
 @pytest.mark.parametrize("kt", ["u32", "u64"])
 def test_radix_patterns(pattern, kt, radix_algo):
