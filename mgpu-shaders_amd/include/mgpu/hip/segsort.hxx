@@ -252,6 +252,9 @@ __global__ __launch_bounds__(segsort_radix_nt) void kernel_segsort_radix_scatter
   }
 }
 
+// Branches of one call (see segmented_sort).
+constexpr int segsort_branches = 4;
+
 // Persistent grids: enough blocks to fill the chip, never more than the work.
 // The blocksort buckets launch segsort_grid_threads / NT blocks (8192 blocks
 // of 64 threads: small shapes need many resident blocks to hide latency).
@@ -393,16 +396,14 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
                     max_tiles);
   if (e != hipSuccess) return hip_status(e);
 
-  // The buckets and the large segments hold disjoint segments: each runs on
-  // a branch of its own, so one bucket's tail overlaps the others' work.
-  fork_join_t fork(stream, NB + 1);
-
-  // Blocksort buckets.
-  for (int b = 0; b < NB; ++b) {
-    e = segsort_dispatch_shape(tuning.shape[b], keys, vals, count, segments, S, list,
-                               bounds + 2 * b, comp, fork.branch(b));
-    if (e != hipSuccess) return hip_status(e);
-  }
+  // The buckets and the large segments hold disjoint segments and run on
+  // branches of their own: branch 0 takes the large-segment radix (a chain of
+  // 4 x (count, scan, scatter) launches, submitted first so that it overlaps
+  // the buckets instead of queueing behind them), branches 1-3 the buckets,
+  // largest capacity first, dealt in turn.  Four branches, as a process has
+  // four hardware queues by default: kernels of streams that share a queue
+  // run in submission order.
+  fork_join_t fork(stream, segsort_branches);
 
   // Large segments: segmented LSD radix, ping-pong keys <-> keys2.
   constexpr int D = (int)sizeof(key_t);
@@ -415,7 +416,7 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   // at the start of the scan aux; the count kernel zeroes them.
   uint64_t* scan_status = (uint64_t*)p;
   const int64_t scan_words = 2 * div_up(max_counts, (int64_t)(256 * 16));
-  const hipStream_t rs = fork.branch(NB);
+  const hipStream_t rs = fork.branch(0);
   for (int pass = 0; pass < D; ++pass) {
     // Every count a task owns is rewritten each pass; entries past the last
     // task only follow the real ones in the scan, so they need no clearing.
@@ -438,6 +439,14 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
     if (e != hipSuccess) return hip_status(e);
     std::swap(k_in, k_out);
     std::swap(v_in, v_out);
+  }
+
+  // Blocksort buckets.
+  for (int i = 0; i < NB; ++i) {
+    const int b = NB - 1 - i;
+    e = segsort_dispatch_shape(tuning.shape[b], keys, vals, count, segments, S, list,
+                               bounds + 2 * b, comp, fork.branch(1 + i % (segsort_branches - 1)));
+    if (e != hipSuccess) return hip_status(e);
   }
   return hip_status(fork.join());
 }
