@@ -603,17 +603,33 @@ void kernel_mergesort_pass4(
 }
 
 // Samples of the largest four-way pass of a sort of `count` keys in tiles of
-// nv (both sides, all groups).
+// nv (both sides, all groups), over both schedules: runs that start at a
+// tile, and runs that start at half a tile (mergesort_half_tiles).
 inline int64_t mergesort_max_samples(int64_t count, int64_t nv) {
-  const int num_passes = find_log2(div_up(count, nv), true);
   int64_t most = 0;
-  for (int level = num_passes & 1; level < num_passes; level += 2) {
-    const int64_t run = nv << level;
-    const int64_t per_side =
-        div_up(count, 4 * run) * merge4_samples_per_group(run, nv / merge4_sample_div);
-    most = max_(most, 2 * per_side);
+  for (const int64_t run0 : {nv, nv / 2}) {
+    const int levels = find_log2(div_up(count, run0), true);
+    for (int level = levels & 1; level < levels; level += 2) {
+      const int64_t run = run0 << level;
+      const int64_t per_side =
+          div_up(count, 4 * run) * merge4_samples_per_group(run, nv / merge4_sample_div);
+      most = max_(most, 2 * per_side);
+    }
   }
   return most;
+}
+
+// Whether a sort with four-way passes starts from runs of half a tile: when
+// runs of a whole tile leave an odd number of merge levels (one two-way pass
+// plus four-way passes), a blocksort of NT / 2 x VT makes the number even
+// (four-way passes only, the same number of passes).  The smaller blocksort
+// is the cheaper one (2^28 u32 pairs: 256 x 19 1.69 ms vs 512 x 19 2.08).
+template <int NT>
+inline bool mergesort_half_tiles(bool four_way, int64_t count, int64_t nv) {
+  if (NT < 2 * warp_size || !four_way) return false;
+  const int levels = find_log2(div_up(count, nv), true);
+  const int half_levels = find_log2(div_up(count, nv / 2), true);
+  return (levels & 1) && !(half_levels & 1);
 }
 
 // Bytes of aux a mergesort call needs (vk/mergesort.hxx:35-41): the
@@ -659,11 +675,14 @@ mgpu_status_t mergesort_kv_impl(void* aux_data, size_t& aux_size, hipStream_t st
     return hip_status(e);
   }
   // Levels of two-way merging: all as two-way passes for small sorts, else
-  // one two-way pass first when their number is odd, then four-way passes of
-  // two levels each.
+  // four-way passes of two levels each, after one two-way pass or from runs
+  // of half a tile when the number of levels is odd.
   const bool four_way = mergesort_four_way<key_t, val_t>(count);
-  const int two_way_passes = four_way ? num_passes & 1 : num_passes;
-  const int data_passes = two_way_passes + (num_passes - two_way_passes) / 2;
+  const bool half = mergesort_half_tiles<NT>(four_way, count, NV);
+  const int64_t run0 = half ? NV / 2 : NV;
+  const int levels = half ? find_log2(div_up(count, run0), true) : num_passes;
+  const int two_way_passes = four_way ? levels & 1 : levels;
+  const int data_passes = two_way_passes + (levels - two_way_passes) / 2;
   void* p = aux_data;
   const int64_t num_partitions = num_ctas + 1;
   int4* mp4 = advance_pointer<int4>(p, num_partitions);
@@ -673,9 +692,17 @@ mgpu_status_t mergesort_kv_impl(void* aux_data, size_t& aux_size, hipStream_t st
   val_t* vals2 = has_values ? advance_pointer<val_t>(p, count) : nullptr;
   key_t* keys_bs = (1 & data_passes) ? keys2 : keys;
   val_t* vals_bs = (1 & data_passes) ? vals2 : vals;
-  e = launch_kernel("mgpu_blocksort", kernel_blocksort<NT, VT, sort_indices, key_t, val_t, comp_t>,
-                    dim3((unsigned)num_ctas), dim3(NT), 0, stream, (const key_t*)keys,
-                    (const val_t*)vals, keys_bs, vals_bs, count, comp);
+  if constexpr (NT >= 2 * warp_size) {
+    if (half)
+      e = launch_kernel("mgpu_blocksort",
+                        kernel_blocksort<NT / 2, VT, sort_indices, key_t, val_t, comp_t>,
+                        dim3((unsigned)div_up(count, run0)), dim3(NT / 2), 0, stream,
+                        (const key_t*)keys, (const val_t*)vals, keys_bs, vals_bs, count, comp);
+  }
+  if (!half)
+    e = launch_kernel("mgpu_blocksort", kernel_blocksort<NT, VT, sort_indices, key_t, val_t, comp_t>,
+                      dim3((unsigned)num_ctas), dim3(NT), 0, stream, (const key_t*)keys,
+                      (const val_t*)vals, keys_bs, vals_bs, count, comp);
   if (e != hipSuccess) return hip_status(e);
   key_t* k_in = keys_bs;
   val_t* v_in = vals_bs;
@@ -697,8 +724,8 @@ mgpu_status_t mergesort_kv_impl(void* aux_data, size_t& aux_size, hipStream_t st
     std::swap(k_in, k_out);
     std::swap(v_in, v_out);
   }
-  for (; level < num_passes; level += 2) {
-    const int64_t run = (int64_t)NV << level;
+  for (; level < levels; level += 2) {
+    const int64_t run = run0 << level;
     constexpr int64_t S = NV / merge4_sample_div;
     const int64_t per_side = div_up(count, 4 * run) * merge4_samples_per_group(run, S);
     e = launch_kernel("mgpu_mergesort_samples", kernel_mergesort_samples4<PNT, key_t, comp_t>,

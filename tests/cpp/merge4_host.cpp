@@ -5,8 +5,8 @@
 //    outputs of the stable four-way merge (ties to the earlier run);
 //  * the two-kernel form used by the four-way passes (merge4_sample for every
 //    sample, then merge4_partition for every tile boundary) over whole sort
-//    schedules (runs of tile << level, partial last groups), at several sample
-//    spacings.
+//    schedules (runs of tile << level, or of half a tile << level, partial
+//    last groups), at several sample spacings.
 // Prints "ALL PASS" and exits 0 when every check holds.
 #include <algorithm>
 #include <cstdio>
@@ -60,16 +60,19 @@ static long check_path(std::mt19937_64& rng, int trial, comp_t comp) {
   return checks;
 }
 
+// With half, the first runs are half a tile long (the merge sort's schedule
+// when a half-size blocksort makes the number of levels even).
 template <typename comp_t>
 static long check_schedule(std::mt19937_64& rng, int64_t count, int64_t nv, int div,
-                           uint32_t range, comp_t comp) {
+                           uint32_t range, comp_t comp, bool half = false) {
   std::vector<uint32_t> keys(count + 1);
   for (int64_t i = 0; i < count; ++i) keys[i] = (uint32_t)(rng() % range);
   const int64_t tiles = div_up(count, nv);
-  const int levels = find_log2(tiles, true);
+  const int64_t run0 = half ? nv / 2 : nv;
+  const int levels = find_log2(div_up(count, run0), true);
   long checks = 0;
   for (int level = levels & 1; level < levels; level += 2) {
-    const int64_t run = nv << level;
+    const int64_t run = run0 << level;
     for (int64_t b = 0; b < count; b += run)
       std::stable_sort(keys.begin() + b, keys.begin() + std::min(count, b + run), comp);
     const int64_t S = nv / div;
@@ -113,6 +116,8 @@ int main() {
     const int div = 1 << (trial % 4);
     checks += trial & 1 ? check_schedule(rng, count, nv, div, range, less_t<uint32_t>())
                         : check_schedule(rng, count, nv, div, range, greater_t<uint32_t>());
+    checks += trial & 1 ? check_schedule(rng, count, nv, div, range, greater_t<uint32_t>(), true)
+                        : check_schedule(rng, count, nv, div, range, less_t<uint32_t>(), true);
   }
   std::printf("%ld checks, %ld failures\n", checks, fails);
   if (fails) return 1;
