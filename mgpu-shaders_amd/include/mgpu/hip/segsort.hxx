@@ -461,10 +461,11 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
 // trial is the whole call on synthetic segments (about 2^22 SplitMix64
 // keys, the same number of keys in every octave of length from 2 to twice
 // the largest capacity); each bucket in turn tries every shape that holds
-// its capacity (up to 8x it) with the other buckets fixed, keeping the
-// fastest (median of 3), two rounds.  Returns the table (also usable as an explicit argument);
-// ms_table[b][shape] is the call's time with that shape in bucket b (last
-// round), -1 for shapes not tried.
+// its capacity (up to 8x it) with the other buckets fixed and takes one
+// that makes the call at least 1.5% faster (median of 5), two rounds.
+// Returns the table (also usable as an explicit argument); ms_table[b][shape]
+// is the call's time with that shape in bucket b (last round), -1 for shapes
+// not tried.
 
 template <typename key_t>
 __global__ __launch_bounds__(256) void kernel_segsort_tune_fill(key_t* keys, uint32_t* vals,
@@ -479,6 +480,8 @@ __global__ __launch_bounds__(256) void kernel_segsort_tune_fill(key_t* keys, uin
     if (vals) vals[i] = (uint32_t)i;
   }
 }
+
+constexpr double segsort_tune_margin = 0.015;
 
 template <typename key_t, typename val_t, typename comp_t>
 inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_tuning_t& out,
@@ -544,10 +547,11 @@ inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_t
     return status;
   }
   check(hipMemcpy(segs, h_segs.data(), S * sizeof(int32_t), hipMemcpyHostToDevice));
-  // One trial: the median of three timed calls on freshly filled keys.
+  // One trial: the median of five timed calls on freshly filled keys.
+  constexpr int reps = 5;
   auto trial = [&](const segsort_tuning_t& t) -> double {
-    float ms[3];
-    for (int rep = 0; rep < 4 && status == MGPU_OK; ++rep) {
+    float ms[reps];
+    for (int rep = 0; rep <= reps && status == MGPU_OK; ++rep) {
       hipLaunchKernelGGL(kernel_segsort_tune_fill<key_t>, dim3(1024), dim3(256), 0, stream, keys,
                          has_values ? vals : nullptr, n, (uint64_t)rep);
       check(hipEventRecord(e0, stream));
@@ -563,8 +567,8 @@ inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_t
       if (rep) ms[rep - 1] = x;  // the first call warms up
     }
     if (status != MGPU_OK) return 1e30;
-    std::sort(ms, ms + 3);
-    return ms[1];
+    std::sort(ms, ms + reps);
+    return ms[reps / 2];
   };
   double best = trial(out);
   for (int round = 0; round < 2 && status == MGPU_OK; ++round) {
@@ -579,7 +583,10 @@ inline mgpu_status_t segsort_autotune(hipStream_t stream, comp_t comp, segsort_t
           segsort_tuning_t t = out;
           t.shape[b] = sh;
           ms = trial(t);
-          if (ms < best) {
+          // A shape replaces the current one only when the call is clearly
+          // faster: buckets that hold few keys move the call's time by less
+          // than the run-to-run spread, and noise must not pick for them.
+          if (ms < best * (1.0 - segsort_tune_margin)) {
             best = ms;
             out.shape[b] = sh;
           }
