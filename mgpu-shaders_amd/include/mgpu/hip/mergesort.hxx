@@ -124,36 +124,41 @@ __device__ inline void block_merge_pass(key_t (&k)[VT], val_t (&v)[VT], int tid,
   __syncthreads();
 }
 
-// Sort tile_count (<= NT*VT) keys (and values) of one tile in LDS and write
-// them out: the body of the blocksort kernel (common/kernel_mergesort.hxx:
-// 12-67), shared with the segmented sort.  With sort_indices the values are
-// index_base + position (common/kernel_mergesort.hxx:43-49).  keys_in may
-// equal keys_out (the tile is fully read before it is written).
+// Loads of one tile (tile_count >= 1) into registers in strided order (slot
+// i of thread tid is element i * NT + tid).  Slots past the end hold 0 or a
+// copy of the last element; the out-of-range mask keeps them out of the sort.
+// All loads are in flight together.
+template <int NT, int VT, bool load_values, typename key_t, typename val_t>
+__device__ inline void blocksort_load(const key_t* keys_in, const val_t* vals_in, int tile_count,
+                                      key_t (&kx)[VT], val_t (&vx)[VT]) {
+  tile_load_strided<NT, VT>(keys_in, tile_count, kx);
+  if constexpr (load_values) tile_load_strided<NT, VT>(vals_in, tile_count, vx);
+}
+
+// Writes a tile loaded by blocksort_load to LDS, in the same strided order.
+template <int NT, int VT, bool load_values, typename key_t, typename val_t>
+__device__ inline void blocksort_stage(const key_t (&kx)[VT], const val_t (&vx)[VT],
+                                       blocksort_storage_t<NT, VT, key_t, val_t>& s) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < VT; ++i) s.keys[i * NT + tid] = kx[i];
+  if constexpr (load_values) {
+#pragma unroll
+    for (int i = 0; i < VT; ++i) s.vals[i * NT + tid] = vx[i];
+  }
+}
+
+// The blocksort of a tile of tile_count (1 <= tile_count <= NT*VT) keys (and
+// values) that blocksort_stage has written to LDS: sort, then write the tile
+// out.  With sort_indices the values are index_base + position
+// (common/kernel_mergesort.hxx:43-49) and none were staged.
 template <int NT, int VT, bool sort_indices, typename key_t, typename val_t, typename comp_t>
-__device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in, key_t* keys_out,
-                                     val_t* vals_out, int tile_count, int64_t index_base,
-                                     comp_t comp, blocksort_storage_t<NT, VT, key_t, val_t>& s) {
+__device__ inline void cta_blocksort_staged(key_t* keys_out, val_t* vals_out, int tile_count,
+                                            int64_t index_base, comp_t comp,
+                                            blocksort_storage_t<NT, VT, key_t, val_t>& s) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   constexpr int NV = NT * VT;
   const int tid = threadIdx.x;
-  if (tile_count <= 0) return;
-  // Unguarded loads of a clamped index (slots past the end get a copy of the
-  // last key; the out-of-range mask keeps them out of the sort), so that all
-  // loads are in flight together instead of one wait per guarded load.
-  {
-    key_t x[VT];
-#pragma unroll
-    for (int i = 0; i < VT; ++i) x[i] = keys_in[min_(i * NT + tid, tile_count - 1)];
-#pragma unroll
-    for (int i = 0; i < VT; ++i) s.keys[i * NT + tid] = x[i];
-  }
-  if constexpr (has_values && !sort_indices) {
-    val_t x[VT];
-#pragma unroll
-    for (int i = 0; i < VT; ++i) x[i] = vals_in[min_(i * NT + tid, tile_count - 1)];
-#pragma unroll
-    for (int i = 0; i < VT; ++i) s.vals[i * NT + tid] = x[i];
-  }
   __syncthreads();
   key_t k[VT];
   val_t v[VT];
@@ -190,20 +195,35 @@ __device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in,
     for (int i = 0; i < VT; ++i) s.vals[VT * tid + i] = v[i];
   }
   __syncthreads();
+  {
+    key_t x[VT];
 #pragma unroll
-  for (int i = 0; i < VT; ++i) {
-    const int j = i * NT + tid;
-    const key_t x = s.keys[j];
-    if (j < tile_count) keys_out[j] = x;
+    for (int i = 0; i < VT; ++i) x[i] = s.keys[i * NT + tid];
+    tile_store_strided<NT, VT>(keys_out, tile_count, x);
   }
   if constexpr (has_values) {
+    val_t x[VT];
 #pragma unroll
-    for (int i = 0; i < VT; ++i) {
-      const int j = i * NT + tid;
-      const val_t x = s.vals[j];
-      if (j < tile_count) vals_out[j] = x;
-    }
+    for (int i = 0; i < VT; ++i) x[i] = s.vals[i * NT + tid];
+    tile_store_strided<NT, VT>(vals_out, tile_count, x);
   }
+}
+
+// Sort tile_count (<= NT*VT) keys (and values) of one tile in LDS and write
+// them out: the body of the blocksort kernel (common/kernel_mergesort.hxx:
+// 12-67).  keys_in may equal keys_out (the tile is fully read before it is
+// written).
+template <int NT, int VT, bool sort_indices, typename key_t, typename val_t, typename comp_t>
+__device__ inline void cta_blocksort(const key_t* keys_in, const val_t* vals_in, key_t* keys_out,
+                                     val_t* vals_out, int tile_count, int64_t index_base,
+                                     comp_t comp, blocksort_storage_t<NT, VT, key_t, val_t>& s) {
+  constexpr bool stage_values = !std::is_same_v<val_t, empty_t> && !sort_indices;
+  if (tile_count <= 0) return;
+  key_t kx[VT];
+  val_t vx[VT];
+  blocksort_load<NT, VT, stage_values>(keys_in, vals_in, tile_count, kx, vx);
+  blocksort_stage<NT, VT, stage_values>(kx, vx, s);
+  cta_blocksort_staged<NT, VT, sort_indices>(keys_out, vals_out, tile_count, index_base, comp, s);
 }
 
 // common/kernel_mergesort.hxx:12-67.

@@ -349,6 +349,72 @@ inline mgpu_status_t init() {
 }
 
 ////////////////////////////////////////////////////////////////////////////////
+// Tiles through raw buffer instructions.  A buffer resource covers
+// [p, p + bytes): loads past the end read 0 and stores past the end are
+// dropped by the hardware's range check, so a tile's edge needs neither a
+// branch per element nor a clamped address, and the per-element offsets are
+// 32-bit (constant parts fold into the instruction's offset field).  Without
+// branches around the stores the compiler also knows how many memory
+// operations are in flight, so waiting for a load does not wait for later
+// stores.  Word 3 (0x00020000) is the gfx9 raw-buffer format word.
+
+__device__ inline __amdgpu_buffer_rsrc_t make_buffer(const void* p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// Elements of 4 or 8 bytes go through buffers; any other size takes the
+// guarded global path.
+template <typename T>
+constexpr bool buffer_element = sizeof(T) == 4 || sizeof(T) == 8;
+
+template <typename T>
+__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t offset) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)offset, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)offset, 0, 0));
+}
+
+template <typename T>
+__device__ inline void buffer_store(T x, __amdgpu_buffer_rsrc_t r, uint32_t offset) {
+  typedef __attribute__((ext_vector_type(2))) unsigned int u32x2;
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)offset, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)offset, 0, 0);
+}
+
+// x[i] = p[i * NT + tid] for the tile's count (>= 1) elements.  Slots past
+// the end hold 0 (buffers) or a copy of the last element (other sizes).
+template <int NT, int VT, typename T>
+__device__ inline void tile_load_strided(const T* p, int count, T (&x)[VT]) {
+  const int tid = threadIdx.x;
+  if constexpr (buffer_element<T>) {
+    const __amdgpu_buffer_rsrc_t r = make_buffer(p, (int64_t)count * sizeof(T));
+#pragma unroll
+    for (int i = 0; i < VT; ++i) x[i] = buffer_load<T>(r, (uint32_t)((i * NT + tid) * sizeof(T)));
+  } else {
+#pragma unroll
+    for (int i = 0; i < VT; ++i) x[i] = p[min_(i * NT + tid, count - 1)];
+  }
+}
+
+// p[i * NT + tid] = x[i] for the slots below count.
+template <int NT, int VT, typename T>
+__device__ inline void tile_store_strided(T* p, int count, const T (&x)[VT]) {
+  const int tid = threadIdx.x;
+  if constexpr (buffer_element<T>) {
+    const __amdgpu_buffer_rsrc_t r = make_buffer(p, (int64_t)count * sizeof(T));
+#pragma unroll
+    for (int i = 0; i < VT; ++i) buffer_store<T>(x[i], r, (uint32_t)((i * NT + tid) * sizeof(T)));
+  } else {
+#pragma unroll
+    for (int i = 0; i < VT; ++i)
+      if (i * NT + tid < count) p[i * NT + tid] = x[i];
+  }
+}
+
+////////////////////////////////////////////////////////////////////////////////
 // wave64 primitives.
 
 __device__ inline int lane_id() { return (int)__lane_id(); }

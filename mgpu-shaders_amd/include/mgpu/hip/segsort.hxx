@@ -97,13 +97,23 @@ inline segsort_tuning_t segsort_default_tuning() {
 
 struct segsort_bucket_caps_t { int cap[segsort_max_buckets]; int num_buckets; };
 
+// A read of device data that no kernel of the call writes while it runs,
+// through the constant address space: where the index is uniform over the
+// wave the load becomes a scalar load, which is counted apart from the vector
+// memory operations, so waiting for it does not also wait for the stores and
+// loads in flight before it.
+template <typename T>
+__device__ inline T read_const(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // Segment i is [begin, end) with both ends clamped to [0, count] and
 // end >= begin: a start at or beyond count gives an empty segment, as in the
 // segmented scan, and no start can place a segment outside the arrays.
 __device__ inline void segment_bounds(const int32_t* segments, int64_t num_segments, int64_t count,
                                       int64_t i, int64_t& begin, int64_t& end) {
-  begin = min_<int64_t>(max_<int64_t>(segments[i], 0), count);
-  const int64_t next = i + 1 < num_segments ? (int64_t)segments[i + 1] : count;
+  begin = min_<int64_t>(max_<int64_t>(read_const(segments, i), 0), count);
+  const int64_t next = i + 1 < num_segments ? (int64_t)read_const(segments, i + 1) : count;
   end = min_<int64_t>(max_<int64_t>(next, begin), count);
 }
 
@@ -168,10 +178,10 @@ __global__ __launch_bounds__(NT) void kernel_segsort_blocks(
     const int32_t* list, const int32_t* bounds, comp_t comp) {
   constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
   __shared__ blocksort_storage_t<NT, VT, key_t, val_t> s;
-  const int32_t first = bounds[0], last = bounds[1];
+  const int64_t first = read_const(bounds, 0), last = read_const(bounds, 1);
   for (int64_t e = first + (int64_t)blockIdx.x; e < last; e += gridDim.x) {
     int64_t b0, b1;
-    segment_bounds(segments, num_segments, count, list[e], b0, b1);
+    segment_bounds(segments, num_segments, count, read_const(list, e), b0, b1);
     val_t* v = has_values ? vals + b0 : vals;
     cta_blocksort<NT, VT, false>(keys + b0, v, keys + b0, v, (int)(b1 - b0), 0, comp, s);
     __syncthreads();
