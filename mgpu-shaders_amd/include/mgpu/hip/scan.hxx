@@ -695,6 +695,24 @@ __global__ __launch_bounds__(NT) void kernel_scan64_lookback(const T* input, T* 
 
 constexpr int scan64_nt = 256, scan64_vt = 48;
 
+// The kernel of scan64, for a caller that has already zeroed the status words
+// (two 64-bit words per scan64_nt * scan64_vt tile).  Arguments are not
+// checked again.
+template <typename T>
+hipError_t scan64_launch_cleared(hipStream_t stream, const T* input, T* output, int64_t count,
+                                 mgpu_scan_type_t type, T init, uint64_t* status) {
+  constexpr int NT = scan64_nt, VT = scan64_vt, NV = NT * VT;
+  const int64_t num_tiles = div_up(count, (int64_t)NV);
+  const lookback_config_t lb = lookback_config(stream);
+  if (type == MGPU_SCAN_INCLUSIVE)
+    return launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, true, T>,
+                         dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
+                         init, status, lb);
+  return launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, false, T>,
+                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,
+                       status, lb);
+}
+
 // Scan of 64-bit integers (uint64_t / int64_t, wrapping modulo 2^64); input
 // and output may alias.
 template <typename T>
@@ -717,16 +735,7 @@ mgpu_status_t scan64(void* aux_data, size_t& aux_size, hipStream_t stream, const
   uint64_t* status = (uint64_t*)aux_data;
   hipError_t e = fill_u32((uint32_t*)status, 4 * num_tiles, 0u, stream);
   if (e != hipSuccess) return hip_status(e);
-  const lookback_config_t lb = lookback_config(stream);
-  if (type == MGPU_SCAN_INCLUSIVE)
-    e = launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, true, T>,
-                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,
-                      status, lb);
-  else
-    e = launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, false, T>,
-                      dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,
-                      status, lb);
-  return hip_status(e);
+  return hip_status(scan64_launch_cleared<T>(stream, input, output, count, type, init, status));
 }
 
 }  // namespace hip

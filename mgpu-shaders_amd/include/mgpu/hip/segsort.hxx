@@ -118,11 +118,16 @@ __device__ inline void segment_bounds(const int32_t* segments, int64_t num_segme
 }
 
 // 1. flags[b * S + i] for b in [0, num_buckets] (num_buckets = large), and
-//    large_tiles[i] = tiles of segment i if it is large, else 0.
+//    large_tiles[i] = tiles of segment i if it is large, else 0.  The grid
+//    also zeroes clear[0, clear_words) (the status words of the two scans
+//    that follow), which saves those scans their own clearing launches.
 template <int NT>
 __global__ __launch_bounds__(NT) void kernel_segsort_classify(
     const int32_t* segments, int64_t num_segments, int64_t count, segsort_bucket_caps_t caps,
-    uint32_t* flags, uint64_t* large_tiles) {
+    uint32_t* flags, uint64_t* large_tiles, uint32_t* clear, int64_t clear_words) {
+  for (int64_t j = (int64_t)blockIdx.x * NT + threadIdx.x; j < clear_words;
+       j += (int64_t)gridDim.x * NT)
+    clear[j] = 0;
   const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (i >= num_segments) return;
   int64_t b0, b1;
@@ -277,6 +282,16 @@ inline int64_t segsort_max_tiles(int64_t count, const segsort_tuning_t& t) {
   return div_up(count, (int64_t)segsort_radix_nv) + count / (t.cap[t.num_buckets - 1] + 1) + 1;
 }
 
+// Status words of the flag scan (one 64-bit word per 256 x 16 tile of the
+// B * S flags, rounded up to an even count) followed by those of the 64-bit
+// tile-position scan (two per scan64 tile of the S segments).
+inline int64_t segsort_flag_scan_words(int64_t S, int64_t B) {
+  return 2 * div_up(div_up(B * S, (int64_t)(256 * 16)), (int64_t)2);
+}
+inline int64_t segsort_scan_status_words(int64_t S, int64_t B) {
+  return segsort_flag_scan_words(S, B) + 2 * div_up(S, (int64_t)(scan64_nt * scan64_vt));
+}
+
 template <typename key_t, typename val_t>
 inline size_t segmented_sort_aux_bytes(int64_t count, int64_t num_segments,
                                        const segsort_tuning_t& tuning) {
@@ -292,15 +307,13 @@ inline size_t segmented_sort_aux_bytes(int64_t count, int64_t num_segments,
   need += aux_bytes<int2>(max_tiles);             // tasks
   need += aux_bytes<int32_t>(1);                  // number of tasks
   need += aux_bytes<uint32_t>(max_counts);        // counts / offsets
+  need += aux_bytes<uint64_t>(segsort_scan_status_words(S, B));  // status of the two scans
   need += aux_bytes<key_t>(count);                // radix ping-pong keys
   if (has_values) need += aux_bytes<uint32_t>(count);
-  size_t scan_need = 0;
+  size_t scan_need = 0;  // the status words of the large-segment counts scan
   scan<256, 16>(nullptr, scan_need, nullptr, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                max(B * S, max_counts), MGPU_SCAN_EXCLUSIVE, 0u);
-  size_t scan64_need = 0;
-  scan64<uint64_t>(nullptr, scan64_need, nullptr, (const uint64_t*)nullptr, (uint64_t*)nullptr, S,
-                   MGPU_SCAN_EXCLUSIVE, (uint64_t)0);
-  return need + max(scan_need, scan64_need);
+                max_counts, MGPU_SCAN_EXCLUSIVE, 0u);
+  return need + scan_need;
 }
 
 template <int SHAPE, typename key_t, typename val_t, typename comp_t>
@@ -379,9 +392,10 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   int2* tasks = advance_pointer<int2>(p, max_tiles);
   int32_t* num_tasks = advance_pointer<int32_t>(p, 1);
   uint32_t* counts = advance_pointer<uint32_t>(p, max_counts);
+  uint64_t* prefix_status = advance_pointer<uint64_t>(p, segsort_scan_status_words(S, B));
+  uint64_t* scan64_status = prefix_status + segsort_flag_scan_words(S, B);
   key_t* keys2 = advance_pointer<key_t>(p, count);
   uint32_t* vals2 = has_values ? advance_pointer<uint32_t>(p, count) : nullptr;
-  size_t scan_cap = aux_size - ((char*)p - (char*)aux_data);
 
   segsort_bucket_caps_t caps;
   caps.num_buckets = NB;
@@ -389,16 +403,17 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
 
   hipError_t e = launch_kernel("mgpu_segsort_classify", kernel_segsort_classify<256>,
                                dim3((unsigned)div_up(S, (int64_t)256)), dim3(256), 0, stream,
-                               segments, S, count, caps, flags, large_tiles);
+                               segments, S, count, caps, flags, large_tiles,
+                               (uint32_t*)prefix_status, 2 * segsort_scan_status_words(S, B));
   if (e != hipSuccess) return hip_status(e);
-  mgpu_status_t st = scan<256, 16>(p, scan_cap, stream, (const uint32_t*)flags, pos, B * S,
-                                   MGPU_SCAN_EXCLUSIVE, 0u);
-  if (st != MGPU_OK) return st;
+  e = scan_launch_cleared<256, 16>(stream, (const uint32_t*)flags, pos, B * S,
+                                   MGPU_SCAN_EXCLUSIVE, 0u, identity_t(), prefix_status);
+  if (e != hipSuccess) return hip_status(e);
   // 64-bit tile positions: their sum cannot wrap even when overlapping
   // segments (starts that do not ascend) claim more tiles than exist.
-  st = scan64<uint64_t>(p, scan_cap, stream, (const uint64_t*)large_tiles, tile_pos, S,
-                        MGPU_SCAN_EXCLUSIVE, (uint64_t)0);
-  if (st != MGPU_OK) return st;
+  e = scan64_launch_cleared<uint64_t>(stream, (const uint64_t*)large_tiles, tile_pos, S,
+                                      MGPU_SCAN_EXCLUSIVE, (uint64_t)0, scan64_status);
+  if (e != hipSuccess) return hip_status(e);
   e = launch_kernel("mgpu_segsort_compact", kernel_segsort_compact<256>,
                     dim3((unsigned)div_up(B * S, (int64_t)256)), dim3(256), 0, stream,
                     (const uint32_t*)flags, (const uint32_t*)pos, S, NB, list, bounds,
