@@ -694,23 +694,37 @@ __global__ __launch_bounds__(NT) void kernel_scan64_lookback(const T* input, T* 
 }
 
 constexpr int scan64_nt = 256, scan64_vt = 48;
+// Scans of up to 2^20 elements take 256 x 8 tiles instead: a few large
+// tiles leave the look-back chain and each tile's loads exposed, more small
+// ones overlap them.
+constexpr int scan64_small_vt = 8;
+constexpr int64_t scan64_small_count = (int64_t)1 << 20;
+inline int64_t scan64_tile_size(int64_t count) {
+  return (int64_t)scan64_nt * (count <= scan64_small_count ? scan64_small_vt : scan64_vt);
+}
 
 // The kernel of scan64, for a caller that has already zeroed the status words
-// (two 64-bit words per scan64_nt * scan64_vt tile).  Arguments are not
+// (two 64-bit words per tile of scan64_tile_size(count) elements).  Arguments are not
 // checked again.
 template <typename T>
 hipError_t scan64_launch_cleared(hipStream_t stream, const T* input, T* output, int64_t count,
                                  mgpu_scan_type_t type, T init, uint64_t* status) {
-  constexpr int NT = scan64_nt, VT = scan64_vt, NV = NT * VT;
-  const int64_t num_tiles = div_up(count, (int64_t)NV);
+  constexpr int NT = scan64_nt;
+  const int64_t num_tiles = div_up(count, scan64_tile_size(count));
   const lookback_config_t lb = lookback_config(stream);
-  if (type == MGPU_SCAN_INCLUSIVE)
-    return launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, true, T>,
+  auto go = [&](auto vt, auto inclusive) {
+    constexpr int VT = decltype(vt)::value;
+    constexpr bool INC = decltype(inclusive)::value;
+    return launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, INC, T>,
                          dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count,
                          init, status, lb);
-  return launch_kernel("mgpu_scan64_lookback", kernel_scan64_lookback<NT, VT, false, T>,
-                       dim3((unsigned)num_tiles), dim3(NT), 0, stream, input, output, count, init,
-                       status, lb);
+  };
+  const bool inc = type == MGPU_SCAN_INCLUSIVE;
+  if (count <= scan64_small_count)
+    return inc ? go(std::integral_constant<int, scan64_small_vt>(), std::true_type())
+               : go(std::integral_constant<int, scan64_small_vt>(), std::false_type());
+  return inc ? go(std::integral_constant<int, scan64_vt>(), std::true_type())
+             : go(std::integral_constant<int, scan64_vt>(), std::false_type());
 }
 
 // Scan of 64-bit integers (uint64_t / int64_t, wrapping modulo 2^64); input
@@ -719,11 +733,10 @@ template <typename T>
 mgpu_status_t scan64(void* aux_data, size_t& aux_size, hipStream_t stream, const T* input,
                      T* output, int64_t count, mgpu_scan_type_t type, T init) {
   static_assert(sizeof(T) == 8, "64-bit element types");
-  constexpr int NT = scan64_nt, VT = scan64_vt, NV = NT * VT;
   if (count < 0) return MGPU_INVALID_ARG;
   if (type != MGPU_SCAN_EXCLUSIVE && type != MGPU_SCAN_INCLUSIVE) return MGPU_INVALID_ARG;
   if (count == 0) return MGPU_OK;
-  const int64_t num_tiles = div_up(count, (int64_t)NV);
+  const int64_t num_tiles = div_up(count, scan64_tile_size(count));
   if (num_tiles >= ((int64_t)1 << 31)) return MGPU_UNSUPPORTED;
   const size_t need = aux_bytes<uint64_t>(2 * num_tiles);
   if (!aux_data) {

@@ -289,7 +289,7 @@ inline int64_t segsort_flag_scan_words(int64_t S, int64_t B) {
   return 2 * div_up(div_up(B * S, (int64_t)(256 * 16)), (int64_t)2);
 }
 inline int64_t segsort_scan_status_words(int64_t S, int64_t B) {
-  return segsort_flag_scan_words(S, B) + 2 * div_up(S, (int64_t)(scan64_nt * scan64_vt));
+  return segsort_flag_scan_words(S, B) + 2 * div_up(S, scan64_tile_size(S));
 }
 
 template <typename key_t, typename val_t>
