@@ -166,7 +166,9 @@ def test_scan_forced_lookback_fallback(limit, inplace):
     assert np.max(np.abs(to_host(outf).astype(np.float64) - exact) / exact) <= F32_SCAN_RTOL
 
 
-U64_NV = 256 * 48  # tile of the 64-bit scan
+U64_NV = 256 * 48  # tile of the 64-bit scan above 2^20 elements
+U64_SMALL_NV = 256 * 8  # its tile up to 2^20 elements
+U64_SMALL_MAX = 1 << 20
 
 
 def u64_expected(x, inclusive, init):
@@ -175,8 +177,9 @@ def u64_expected(x, inclusive, init):
     return inc if inclusive else np.concatenate([np.array([init], np.uint64), inc[:-1]])
 
 
-@pytest.mark.parametrize("n", [1, 7, U64_NV - 1, U64_NV, U64_NV + 1, 3 * U64_NV + 5,
-                               300 * U64_NV + 1, (1 << 22) + 1])
+@pytest.mark.parametrize("n", [1, 7, U64_SMALL_NV - 1, U64_SMALL_NV, U64_SMALL_NV + 1,
+                               U64_NV - 1, U64_NV, U64_NV + 1, 3 * U64_NV + 5,
+                               U64_SMALL_MAX, U64_SMALL_MAX + 1, 300 * U64_NV + 1, (1 << 22) + 1])
 @pytest.mark.parametrize("inclusive", [False, True])
 def test_scan_u64(n, inclusive):
     banner("scan u64", n, inclusive)
@@ -193,12 +196,12 @@ def test_scan_u64(n, inclusive):
 
 @pytest.mark.parametrize("limit", [0, 1, 512])
 @pytest.mark.parametrize("in_place", [False, True])
-def test_scan_u64_lookback_fallback_and_in_place(limit, in_place):
+@pytest.mark.parametrize("n", [300 * U64_SMALL_NV + 5, 700 * U64_NV + 17])
+def test_scan_u64_lookback_fallback_and_in_place(limit, in_place, n):
     """A tiny spin limit makes tiles sum late predecessors from their input
     (in place: with the re-check of the predecessor's words); the result is
-    unchanged."""
-    banner("scan u64 fallback", limit, in_place)
-    n = 700 * U64_NV + 17
+    unchanged, with the small tiles and the large ones."""
+    banner("scan u64 fallback", limit, in_place, n)
     x = oracle.generate(n, "u64", 31)
     old = mgpu_hip.lookback_spin_limit()
     try:
