@@ -5,7 +5,10 @@
 
 // Merge / mergesort tiles, measured on MI355X with 32-bit keys: merge of
 // 2 x 2^27 keys 0.404 ms at 512 x 23 (256 x 15: 0.489); mergesort of 2^28
-// key-value pairs 13.6 ms at 512 x 19 (256 x 15: 16.2, 512 x 11: 14.3).
+// key-value pairs 13.6 ms at 512 x 19 (256 x 15: 16.2, 512 x 11: 14.3) with two-way
+// passes; with four-way passes 9.47-9.51 ms at 512 x 19 (1024 x 9: 9.42-9.50, 512 x 13:
+// 9.55, 512 x 17: 9.60, 256 x 19: 9.61, 512 x 11: 9.90, 512 x 15: 10.06, 1024 x 7:
+// 10.12, 256 x 23: 10.27, 1024 x 13: 12.2, 1024 x 11: 12.4, 512 x 21: 14.2).
 // 8-byte keys: merge of 2 x 2^26 u64 keys 0.395 ms at 512 x 15 (256 x 15:
 // 0.435, 256 x 23: 0.421, 512 x 11: 0.427); mergesort of 2^27 u64 pairs
 // 9.80 ms at 512 x 13 (512 x 11: 9.87, 512 x 9: 10.34, 256 x 15: 10.65,
