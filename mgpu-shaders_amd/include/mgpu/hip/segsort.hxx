@@ -424,10 +424,12 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   // The buckets and the large segments hold disjoint segments and run on
   // branches of their own: branch 0 takes the large-segment radix (a chain of
   // 4 x (count, scan, scatter) launches, submitted first so that it overlaps
-  // the buckets instead of queueing behind them), branches 1-3 the buckets,
-  // largest capacity first, dealt in turn.  Four branches, as a process has
-  // four hardware queues by default: kernels of streams that share a queue
-  // run in submission order.
+  // the buckets instead of queueing behind them), branches 1-3 the buckets in
+  // three contiguous groups, largest capacity first: the largest tiles, which
+  // leave a CU room only for small ones, run one after another on branch 1
+  // while the middle and the smallest groups run beside them.  Four
+  // branches, as a process has four hardware queues by default: kernels of
+  // streams that share a queue run in submission order.
   fork_join_t fork(stream, segsort_branches);
 
   // Large segments: segmented LSD radix, ping-pong keys <-> keys2.
@@ -470,7 +472,7 @@ mgpu_status_t segmented_sort(void* aux_data, size_t& aux_size, hipStream_t strea
   for (int i = 0; i < NB; ++i) {
     const int b = NB - 1 - i;
     e = segsort_dispatch_shape(tuning.shape[b], keys, vals, count, segments, S, list,
-                               bounds + 2 * b, comp, fork.branch(1 + i % (segsort_branches - 1)));
+                               bounds + 2 * b, comp, fork.branch(1 + i * (segsort_branches - 1) / NB));
     if (e != hipSuccess) return hip_status(e);
   }
   return hip_status(fork.join());
