@@ -347,7 +347,7 @@ mgpu_status_t merge_path_partitions(hipStream_t stream, const key_t* a, int64_t 
     return MGPU_UNSUPPORTED;
   if (!mp_data || (a_count && !a) || (b_count && !b)) return MGPU_INVALID_ARG;
   const int64_t num_partitions = num_merge_partitions(a_count + b_count, spacing);
-  constexpr int PNT = 256;
+  constexpr int PNT = 64;  // one wave per block: the searches spread over every CU
   return hip_status(launch_kernel("mgpu_merge_partition",
                                   kernel_merge_partition<PNT, bounds, key_t, mp_t, comp_t>,
                                   dim3((unsigned)div_up(num_partitions, (int64_t)PNT)), dim3(PNT),
@@ -489,7 +489,7 @@ mgpu_status_t merge(void* aux_data, size_t& aux_size, hipStream_t stream,
   if (mgpu_status_t st = check_aux(aux_data, aux_size, need)) return st;
   void* p = aux_data;
   int64_t* mp = advance_pointer<int64_t>(p, num_partitions);
-  constexpr int PNT = 256;
+  constexpr int PNT = 64;  // one wave per block: the searches spread over every CU
   hipError_t e = launch_kernel("mgpu_merge_partition",
                                kernel_merge_partition<PNT, bounds_lower, key_t, int64_t, comp_t>,
                                dim3((unsigned)div_up(num_partitions, (int64_t)PNT)),

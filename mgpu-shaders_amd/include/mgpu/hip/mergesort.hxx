@@ -728,7 +728,9 @@ mgpu_status_t mergesort_kv_impl(void* aux_data, size_t& aux_size, hipStream_t st
   val_t* v_in = vals_bs;
   key_t* k_out = (1 & data_passes) ? keys : keys2;
   val_t* v_out = (1 & data_passes) ? vals : vals2;
-  constexpr int PNT = 256;
+  // Partition searches are chains of dependent loads, one per thread: blocks of
+  // one wave spread them over every CU (256-thread blocks left most CUs idle).
+  constexpr int PNT = 64;
   const unsigned partition_grid = (unsigned)div_up(num_partitions, (int64_t)PNT);
   int level = 0;
   for (; level < two_way_passes; ++level) {
