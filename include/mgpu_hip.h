@@ -111,11 +111,12 @@ MGPU_DECL_MERGESORT(i64, int64_t)
 MGPU_DECL_MERGESORT(f64, double)
 /* Merge pass structure (process-wide): 0 = automatic, 2 = two-way passes
  * only (the reference's structure: one merge-path pass per level), 4 =
- * four-way passes (two levels per pass; one two-way pass first when the
- * number of levels is odd) at every size.  Automatic takes four-way passes
- * once a pass moves 256 MB of keys and values (2^28 u32 pairs: 8 global
- * passes instead of 15).  Results are identical; the sizing call returns
- * enough aux for either.  No counterpart in the reference.                  */
+ * four-way passes (two levels per pass; when whole tiles leave an odd number
+ * of levels the blocksort sorts half tiles, so every pass is four-way) at
+ * every size.  Automatic takes four-way passes once a pass moves 168 MiB of
+ * keys and values, or 256 MiB of keys only (2^28 u32 pairs: 8 global passes
+ * instead of 15).  Results are identical; the sizing call returns enough aux
+ * for either.  No counterpart in the reference.                             */
 void mgpu_set_mergesort_passes(int ways);
 int mgpu_mergesort_passes(void);
 

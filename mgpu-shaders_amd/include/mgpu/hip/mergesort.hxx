@@ -293,16 +293,18 @@ __global__ __launch_bounds__(NT) void kernel_mergesort_pass(
 constexpr int merge4_sample_div = 1;
 
 // Four-way passes pay off once a pass moves enough bytes to outweigh the
-// extra partition kernel: per-pass keys + values >= 256 MB (2^24 u32 pairs:
-// 0.81 ms two-way vs 0.92 four-way; 5 x 10^7 pairs 2.51 vs 1.93; 2^28 pairs
-// 13.5 vs 10.1).  The same byte count marks the crossover for 64-bit keys
-// (2^24 u64 + u32 pairs, 201 MB: 1.36 vs 1.44 ms; 2^25 u64 keys, 268 MB:
-// 1.82 vs 1.84; 2^25 pairs, 402 MB: 2.54 vs 2.30; 2^27 pairs 9.75 vs 7.04).
+// extra partition kernels.  Keys only: per-pass keys >= 256 MiB (2^25 u32
+// keys, 134 MB: 0.78 ms two-way vs 1.03 four-way; 201 MB: 1.34 vs 1.47; 2^28
+// keys: 7.4 vs 6.0).  Keys and values: >= 168 MiB per pass (u32 pairs,
+// 168 MB: 1.12 vs 1.14 ms, 201 MB: 1.32 vs 1.26, 2^28 pairs 13.5 vs 9.2; u64
+// key + u32 value pairs, 176 MB: 1.14 vs 1.09, 2^27 pairs 9.75 vs 6.5).
 constexpr int64_t merge4_min_bytes = (int64_t)1 << 28;
+constexpr int64_t merge4_min_bytes_kv = (int64_t)168 << 20;
 
 // Pass structure of the merge sort (a process-wide setting): 0 = automatic
-// (four-way passes once a pass moves merge4_min_bytes), 2 = two-way passes
-// only (the reference's structure), 4 = four-way passes at every size.  The
+// (four-way passes once a pass moves merge4_min_bytes of keys, or
+// merge4_min_bytes_kv of keys and values), 2 = two-way passes only (the
+// reference's structure), 4 = four-way passes at every size.  The
 // sorted result is the same; the sizing call covers every structure.
 inline std::atomic<int>& mergesort_passes_setting() {
   static std::atomic<int> a{0};
@@ -313,8 +315,9 @@ template <typename key_t, typename val_t>
 inline bool mergesort_four_way(int64_t count) {
   const int setting = mergesort_passes_setting().load(std::memory_order_relaxed);
   if (setting == 2 || setting == 4) return setting == 4;
-  return count * (int64_t)(sizeof(key_t) + (std::is_same_v<val_t, empty_t> ? 0 : sizeof(val_t))) >=
-         merge4_min_bytes;
+  constexpr bool has_values = !std::is_same_v<val_t, empty_t>;
+  return count * (int64_t)(sizeof(key_t) + (has_values ? sizeof(val_t) : 0)) >=
+         (has_values ? merge4_min_bytes_kv : merge4_min_bytes);
 }
 
 // Samples per side (X or Y) of one group of four runs of `run` keys.
