@@ -79,6 +79,27 @@ def test_mergesort_kv(n, kt, merge_passes):
     np.testing.assert_array_equal(to_host(dv), ev)
 
 
+@pytest.mark.parametrize("kt,n", [("u64", (1 << 24) + 5), ("u32", (24 << 20) + 3),
+                                  ("u32", (20 << 20) - 7)])
+def test_mergesort_kv_automatic_structure(kt, n):
+    """The automatic pass structure around its key-value threshold (168 MiB
+    per pass: four-way for the first two sizes, two-way for the last), against
+    the oracle's stable sort."""
+    banner("mergesort_kv automatic", kt, n)
+    old = mgpu_hip.mergesort_passes()
+    try:
+        mgpu_hip.set_mergesort_passes(0)
+        keys = make_keys(kt, n, 7, "ties")
+        vals = np.arange(n, dtype=np.uint32)
+        dk, dv = dev_keys(keys), to_dev(vals)
+        mgpu_hip.mergesort_kv(dk, dv)
+    finally:
+        mgpu_hip.set_mergesort_passes(old)
+    ek, ev = oracle.stable_sort(keys, vals, kt)
+    np.testing.assert_array_equal(to_host(dk), ek)
+    np.testing.assert_array_equal(to_host(dv), ev)
+
+
 @pytest.mark.parametrize("n", GROUP_SIZES)
 @pytest.mark.parametrize("kt", ["u32", "f64"])
 @pytest.mark.parametrize("comp", ["less", "greater"])
